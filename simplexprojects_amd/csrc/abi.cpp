@@ -1,0 +1,647 @@
+// abi.cpp -- the C ABI of libblpg.so: contexts, packed device batches, size-bin routing,
+// solve / verify / fallback pipeline and result download.
+//
+// Replaces the host half of the reference's Simplex class (simplex.cu:387-978): instead of
+// building full tableaux on the host with OpenMP and copying them to the device per call
+// (simplex.cu:506-534, 944-953), the batch is packed once as compact (A, b, c, sense)
+// records, the kernels build their tableaux on the device, and only results come back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/blpg/blpg.h"
+#include "blpg_internal.hpp"
+#include "kernels.hpp"
+#include "tuner.hpp"
+
+using namespace blpg;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct blpg_ctx {
+  blpg_options opt;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t props{};
+  std::string err;
+  DevBuf ws;   // HBM slabs of the streamed kernel
+  DevBuf vws;  // LU workspace of the verify kernel
+  Tuner tuner;
+};
+
+namespace {
+
+struct Bin {
+  int kernel = BLPG_KERNEL_STREAM;  // BLPG_KERNEL_*
+  int variant = 0;                  // tile variant
+  int nt = 256;                     // stream threads
+  int rcap = 0, ccap = 0;
+  std::vector<int32_t> ids;
+  int32_t* d_idx = nullptr;
+};
+
+}  // namespace
+
+struct blpg_batch {
+  int64_t count = 0;
+  int precision = 0;
+  std::vector<LpDesc> desc;
+  int64_t pool_elems = 0, spool_elems = 0, x_elems = 0, basis_elems = 0;
+  int mmax = 0, nmax = 0;
+  void* mem = nullptr;
+  size_t mem_bytes = 0;
+  LpDesc* d_desc = nullptr;
+  double* d_pool = nullptr;
+  int8_t* d_spool = nullptr;
+  Out out{};
+  Check chk{};
+  int32_t* d_fb = nullptr;        // fallback-used flag per LP
+  int32_t* d_all = nullptr;       // 0..count-1
+  int32_t* d_fail = nullptr;      // LP ids failing the check
+  int32_t* d_counters = nullptr;  // [0..nbins) queues, then verify/fallback queues and counts
+  std::vector<Bin> bins;
+  blpg_batch_info info{};
+  hipEvent_t ev[40] = {};
+  int nev = 0;
+};
+
+// ------------------------------------------------------------------------------------
+namespace {
+
+int fail(blpg_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(ctx, BLPG_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                                      \
+  } while (0)
+
+void resolve(const blpg_options& o, double& eps_d, double& eps_p, double& tol, double& kkt) {
+  const bool f32 = o.precision == BLPG_FP32;
+  eps_d = o.eps_d > 0 ? o.eps_d : (f32 ? 1e-6 : 1e-9);
+  eps_p = o.eps_p > 0 ? o.eps_p : (f32 ? 1e-6 : 1e-9);
+  tol = o.tol_feas > 0 ? o.tol_feas : (f32 ? 1e-4 : 1e-7);
+  kkt = o.kkt_tol > 0 ? o.kkt_tol : 1e-9;
+}
+
+int bland_after(const blpg_options& o) { return o.bland_after > 0 ? o.bland_after : 50; }
+
+bool finite_all(const double* p, int64_t n) {
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+
+int grow(blpg_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes) return BLPG_OK;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  if (hipMalloc(&b.p, bytes) != hipSuccess)
+    return fail(ctx, BLPG_E_NOMEM, "hipMalloc(%zu) failed", bytes);
+  b.bytes = bytes;
+  return BLPG_OK;
+}
+
+int persistent_grid(const blpg_ctx* ctx, int per_cu, int64_t count) {
+  const int64_t g = (int64_t)ctx->props.multiProcessorCount * std::max(per_cu, 1);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, count));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+extern "C" {
+
+void blpg_default_options(blpg_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->precision = BLPG_FP64;
+  o->verify = 1;
+  o->kernel = BLPG_KERNEL_AUTO;
+  o->use_graph = 1;
+}
+
+const char* blpg_version(void) { return "blpg-mi355x 0.1 (gfx950)"; }
+
+int blpg_create(blpg_ctx** out, const blpg_options* opt) {
+  if (!out) return BLPG_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BLPG_E_NODEVICE;
+  blpg_ctx* ctx = new blpg_ctx();
+  if (opt) ctx->opt = *opt;
+  else blpg_default_options(&ctx->opt);
+  ctx->device = ctx->opt.device;
+  if (ctx->device < 0 || ctx->device >= ndev) { delete ctx; return BLPG_E_NODEVICE; }
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      hipGetDeviceProperties(&ctx->props, ctx->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return BLPG_E_HIP;
+  }
+  ctx->tuner.init(ctx->props);
+  *out = ctx;
+  return BLPG_OK;
+}
+
+void blpg_destroy(blpg_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) { (void)hipStreamSynchronize(ctx->stream); (void)hipStreamDestroy(ctx->stream); }
+  if (ctx->ws.p) (void)hipFree(ctx->ws.p);
+  if (ctx->vws.p) (void)hipFree(ctx->vws.p);
+  delete ctx;
+}
+
+const char* blpg_last_error(const blpg_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int blpg_set_options(blpg_ctx* ctx, const blpg_options* opt) {
+  if (!ctx || !opt) return BLPG_E_INVALID;
+  if (opt->device != ctx->device) return fail(ctx, BLPG_E_INVALID, "device cannot change");
+  ctx->opt = *opt;
+  return BLPG_OK;
+}
+
+int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_batch** out) {
+  if (!ctx || !out || (count > 0 && !lps) || count < 0)
+    return fail(ctx, BLPG_E_INVALID, "invalid arguments to blpg_batch_upload");
+  if (count > INT32_MAX / 2) return fail(ctx, BLPG_E_INVALID, "batch too large");
+  *out = nullptr;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  blpg_batch* B = new blpg_batch();
+  B->count = count;
+  B->precision = ctx->opt.precision;
+  B->desc.resize(count);
+  // pack: distinct A / b / c / sense pointers are stored once (shared-A batches)
+  std::map<const void*, int64_t> seen;
+  std::vector<std::pair<const double*, int64_t>> dsrc;  // (ptr, len) in pool order
+  std::vector<std::pair<const int8_t*, int64_t>> ssrc;
+  auto place = [&](const double* p, int64_t len) -> int64_t {
+    auto it = seen.find(p);
+    if (it != seen.end()) return it->second;
+    const int64_t off = B->pool_elems;
+    B->pool_elems += (len + 7) & ~7ll;
+    dsrc.push_back({p, len});
+    seen[p] = off;
+    return off;
+  };
+  std::map<const void*, int64_t> sseen;
+  for (int64_t k = 0; k < count; ++k) {
+    const blpg_lp& L = lps[k];
+    if (L.m < 1 || L.n < 1 || !L.A || !L.b || !L.c) {
+      delete B;
+      return fail(ctx, BLPG_E_INVALID, "LP %lld: m, n must be >= 1 and A, b, c non-NULL",
+                  (long long)k);
+    }
+    if (L.m > 4096 || L.n > 4096) {
+      delete B;
+      return fail(ctx, BLPG_E_INVALID, "LP %lld: m, n above 4096 not supported", (long long)k);
+    }
+    LpDesc& d = B->desc[k];
+    d.m = L.m;
+    d.n = L.n;
+    if (!seen.count(L.A) && !finite_all(L.A, (int64_t)L.m * L.n)) {
+      delete B;
+      return fail(ctx, BLPG_E_INVALID, "LP %lld: A has NaN/Inf", (long long)k);
+    }
+    if (!seen.count(L.b) && !finite_all(L.b, L.m)) {
+      delete B;
+      return fail(ctx, BLPG_E_INVALID, "LP %lld: b has NaN/Inf", (long long)k);
+    }
+    if (!seen.count(L.c) && !finite_all(L.c, L.n)) {
+      delete B;
+      return fail(ctx, BLPG_E_INVALID, "LP %lld: c has NaN/Inf", (long long)k);
+    }
+    d.a = place(L.A, (int64_t)L.m * L.n);
+    d.b = place(L.b, L.m);
+    d.c = place(L.c, L.n);
+    if (L.sense) {
+      for (int i = 0; i < L.m; ++i)
+        if (L.sense[i] < 0 || L.sense[i] > 2) {
+          delete B;
+          return fail(ctx, BLPG_E_INVALID, "LP %lld: sense[%d] not in {0,1,2}", (long long)k, i);
+        }
+      auto it = sseen.find(L.sense);
+      if (it != sseen.end()) d.s = it->second;
+      else {
+        d.s = B->spool_elems;
+        B->spool_elems += (L.m + 15) & ~15ll;
+        ssrc.push_back({L.sense, L.m});
+        sseen[L.sense] = d.s;
+      }
+    } else {
+      d.s = -1;
+    }
+    d.x = B->x_elems;
+    B->x_elems += L.n;
+    d.basis = B->basis_elems;
+    B->basis_elems += L.m;
+    B->mmax = std::max(B->mmax, L.m);
+    B->nmax = std::max(B->nmax, L.n);
+  }
+
+  // ---- size bins, routed by the tuner ----
+  std::map<std::pair<int, int>, int> binof;  // (kernel/variant key, cap key) -> bin
+  for (int64_t k = 0; k < count; ++k) {
+    const Route r = ctx->tuner.route(B->desc[k].m, B->desc[k].n, ctx->opt.precision,
+                                     ctx->opt.kernel);
+    const std::pair<int, int> key{r.kernel * 1000 + r.variant, r.nt * 100000 + r.rcap * 0 + r.bin};
+    auto it = binof.find(key);
+    int bi;
+    if (it == binof.end()) {
+      bi = (int)B->bins.size();
+      binof[key] = bi;
+      Bin b;
+      b.kernel = r.kernel;
+      b.variant = r.variant;
+      b.nt = r.nt;
+      B->bins.push_back(b);
+    } else {
+      bi = it->second;
+    }
+    Bin& b = B->bins[bi];
+    b.ids.push_back((int32_t)k);
+    b.rcap = std::max(b.rcap, B->desc[k].m + 2);
+    b.ccap = std::max(b.ccap, B->desc[k].n + 2);
+  }
+  if (B->bins.size() > 16) {
+    delete B;
+    return fail(ctx, BLPG_E_INVALID, "more than 16 size bins in one batch");
+  }
+
+  // ---- one device allocation, carved ----
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const int64_t n = count;
+  const int nb = (int)B->bins.size();
+  size_t off = 0;
+  const size_t o_desc = off; off += al(sizeof(LpDesc) * n);
+  const size_t o_pool = off; off += al(sizeof(double) * B->pool_elems);
+  const size_t o_spool = off; off += al(B->spool_elems + 16);
+  const size_t o_i32 = off; off += al(sizeof(int32_t) * n * 9);     // status pivots p1 unb ok fb all fail idx
+  const size_t o_f64 = off; off += al(sizeof(double) * n * 5);      // obj wfin chkobj kktp kktd
+  const size_t o_x = off; off += al(sizeof(double) * B->x_elems * 2);
+  const size_t o_basis = off; off += al(sizeof(int32_t) * B->basis_elems);
+  const size_t o_cnt = off; off += al(sizeof(int32_t) * 64);
+  B->mem_bytes = off;
+  if (ctx->opt.hbm_budget > 0 && (int64_t)off > ctx->opt.hbm_budget) {
+    delete B;
+    return fail(ctx, BLPG_E_BUDGET, "batch needs %zu bytes, budget %lld", off,
+                (long long)ctx->opt.hbm_budget);
+  }
+  if (hipMalloc(&B->mem, off) != hipSuccess) {
+    delete B;
+    return fail(ctx, BLPG_E_NOMEM, "hipMalloc(%zu) for the batch failed", off);
+  }
+  char* base = (char*)B->mem;
+  B->d_desc = (LpDesc*)(base + o_desc);
+  B->d_pool = (double*)(base + o_pool);
+  B->d_spool = (int8_t*)(base + o_spool);
+  int32_t* i32 = (int32_t*)(base + o_i32);
+  B->out.status = i32;
+  B->out.pivots = i32 + n;
+  B->out.p1 = i32 + 2 * n;
+  B->out.unb = i32 + 3 * n;
+  B->chk.ok = i32 + 4 * n;
+  B->d_fb = i32 + 5 * n;
+  B->chk.fb = B->d_fb;
+  B->d_all = i32 + 6 * n;
+  B->d_fail = i32 + 7 * n;
+  int32_t* d_binidx = i32 + 8 * n;
+  double* f64 = (double*)(base + o_f64);
+  B->out.obj = f64;
+  B->out.wfin = f64 + n;
+  B->chk.obj = f64 + 2 * n;
+  B->chk.kkt_p = f64 + 3 * n;
+  B->chk.kkt_d = f64 + 4 * n;
+  B->out.x = (double*)(base + o_x);
+  B->chk.x = B->out.x + B->x_elems;
+  B->out.basis = (int32_t*)(base + o_basis);
+  B->d_counters = (int32_t*)(base + o_cnt);
+
+  // ---- host staging (pinned) and one upload ----
+  const size_t stage_bytes = o_i32 + sizeof(int32_t) * n * 9;
+  char* h = nullptr;
+  if (hipHostMalloc((void**)&h, stage_bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipFree(B->mem);
+    delete B;
+    return fail(ctx, BLPG_E_NOMEM, "hipHostMalloc(%zu) failed", stage_bytes);
+  }
+  std::memcpy(h + o_desc, B->desc.data(), sizeof(LpDesc) * n);
+  for (auto& s : dsrc) {
+    const int64_t o = seen[s.first];
+    std::memcpy(h + o_pool + sizeof(double) * o, s.first, sizeof(double) * s.second);
+  }
+  for (auto& s : ssrc) {
+    const int64_t o = sseen[s.first];
+    std::memcpy(h + o_spool + o, s.first, (size_t)s.second);
+  }
+  int32_t* hi = (int32_t*)(h + o_i32);
+  std::memset(hi, 0, sizeof(int32_t) * n * 8);
+  for (int64_t k = 0; k < n; ++k) hi[6 * n + k] = (int32_t)k;
+  int64_t pos = 0;
+  for (auto& b : B->bins) {
+    b.d_idx = d_binidx + pos;
+    std::memcpy(hi + 8 * n + pos, b.ids.data(), sizeof(int32_t) * b.ids.size());
+    pos += (int64_t)b.ids.size();
+  }
+  hipError_t e = hipMemcpyAsync(B->mem, h, stage_bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipHostFree(h);
+  if (e != hipSuccess) {
+    (void)hipFree(B->mem);
+    delete B;
+    return fail(ctx, BLPG_E_HIP, "batch upload failed: %s", hipGetErrorString(e));
+  }
+  for (int i = 0; i < 40; ++i)
+    if (hipEventCreate(&B->ev[i]) != hipSuccess) {
+      blpg_batch_free(ctx, B);
+      return fail(ctx, BLPG_E_HIP, "hipEventCreate failed");
+    }
+  B->info.nbins = nb;
+  for (int i = 0; i < nb; ++i) {
+    B->info.kernel[i] = B->bins[i].kernel;
+    B->info.rcap[i] = B->bins[i].rcap;
+    B->info.ccap[i] = B->bins[i].ccap;
+    B->info.count[i] = (int64_t)B->bins[i].ids.size();
+  }
+  *out = B;
+  return BLPG_OK;
+}
+
+void blpg_batch_free(blpg_ctx* ctx, blpg_batch* B) {
+  if (!B) return;
+  if (ctx) (void)hipSetDevice(ctx->device);
+  for (int i = 0; i < 40; ++i)
+    if (B->ev[i]) (void)hipEventDestroy(B->ev[i]);
+  if (B->mem) (void)hipFree(B->mem);
+  delete B;
+}
+
+static int launch_bin(blpg_ctx* ctx, blpg_batch* B, const Bin& b, int precision,
+                      const int32_t* idx, int32_t count, int32_t* queue, hipStream_t st) {
+  double eps_d, eps_p, tol, kkt;
+  blpg_options o = ctx->opt;
+  o.precision = precision;
+  resolve(o, eps_d, eps_p, tol, kkt);
+  if (b.kernel == BLPG_KERNEL_TILE) {
+    TileLaunch L{};
+    L.precision = precision;
+    L.variant = b.variant;
+    L.desc = B->d_desc; L.pool = B->d_pool; L.spool = B->d_spool;
+    L.idx = idx; L.count = count; L.queue = queue; L.out = B->out;
+    L.eps_d = eps_d; L.eps_p = eps_p; L.tol = tol;
+    L.max_iter = o.max_iter; L.bland_after = bland_after(o);
+    L.grid = persistent_grid(ctx, tile_max_blocks_per_cu(precision, b.variant), count);
+    HIPCHK(ctx, launch_fill_i32(queue, 0, 1, st));
+    HIPCHK(ctx, launch_tile(L, st));
+    return BLPG_OK;
+  }
+  const bool lds = b.kernel == BLPG_KERNEL_LDS;
+  StreamLaunch L{};
+  L.precision = precision;
+  L.lds_tab = lds;
+  L.nt = b.nt;
+  L.desc = B->d_desc; L.pool = B->d_pool; L.spool = B->d_spool;
+  L.idx = idx; L.count = count; L.queue = queue; L.out = B->out;
+  L.eps_d = eps_d; L.eps_p = eps_p; L.tol = tol;
+  L.max_iter = o.max_iter; L.bland_after = bland_after(o);
+  L.rcap = b.rcap; L.ccap = b.ccap;
+  const int per_cu = stream_max_blocks_per_cu(precision, lds, b.nt, b.rcap, b.ccap);
+  if (per_cu <= 0)
+    return fail(ctx, BLPG_E_INVALID, "kernel cannot be resident (rcap %d ccap %d lds %d)",
+                b.rcap, b.ccap, (int)lds);
+  L.grid = persistent_grid(ctx, per_cu, count);
+  if (!lds) {
+    const size_t esz = precision == BLPG_FP64 ? 8 : 4;
+    L.ws_stride = ((int64_t)b.rcap * b.ccap + 63) & ~63ll;
+    const int rc = grow(ctx, ctx->ws, esz * (size_t)L.ws_stride * L.grid);
+    if (rc) return rc;
+    L.ws = ctx->ws.p;
+  }
+  HIPCHK(ctx, launch_fill_i32(queue, 0, 1, st));
+  HIPCHK(ctx, launch_stream(L, st));
+  return BLPG_OK;
+}
+
+int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
+  if (!ctx || !B) return BLPG_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (B->count == 0) return BLPG_OK;
+  hipStream_t st = ctx->stream;
+  const int precision = ctx->opt.precision;
+  B->precision = precision;
+  B->nev = 0;
+  int32_t* q = B->d_counters;
+  HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));
+  HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
+  for (size_t i = 0; i < B->bins.size(); ++i) {
+    const Bin& b = B->bins[i];
+    HIPCHK(ctx, hipEventRecord(B->ev[1 + 2 * i], st));
+    const int rc = launch_bin(ctx, B, b, precision, b.d_idx, (int32_t)b.ids.size(), q + i, st);
+    if (rc) return rc;
+    HIPCHK(ctx, hipEventRecord(B->ev[2 + 2 * i], st));
+  }
+  B->nev = 1 + 2 * (int)B->bins.size();
+  HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));  // end of solve kernels
+  // FP64 check of the final bases, and the FP64 fallback for FP32 failures
+  double eps_d, eps_p, tol, kkt;
+  resolve(ctx->opt, eps_d, eps_p, tol, kkt);
+  const bool do_verify = ctx->opt.verify || precision == BLPG_FP32;
+  if (do_verify) {
+    VerifyLaunch V{};
+    V.desc = B->d_desc; V.pool = B->d_pool; V.spool = B->d_spool;
+    V.idx = B->d_all; V.count = (int32_t)B->count;
+    V.out = B->out; V.chk = B->chk; V.tol = kkt;
+    V.fp32_source = precision == BLPG_FP32;
+    V.mcap = B->mmax;
+    V.ncap = B->nmax;
+    const bool lds = verify_lds_bytes(B->mmax, B->nmax, true) <= 100 * 1024;
+    V.grid = persistent_grid(ctx, verify_max_blocks_per_cu(B->mmax, B->nmax, lds), B->count);
+    V.ws_stride = ((int64_t)B->mmax * B->mmax + 63) & ~63ll;
+    if (!lds) {
+      const int rc = grow(ctx, ctx->vws, sizeof(double) * (size_t)V.ws_stride * V.grid);
+      if (rc) return rc;
+      V.ws = (double*)ctx->vws.p;
+    }
+    V.queue = q + 20;
+    V.fail_list = B->d_fail;
+    V.fail_count = q + 21;
+    HIPCHK(ctx, launch_fill_i32(q + 20, 0, 2, st));
+    HIPCHK(ctx, launch_verify(V, st));
+    HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));  // end of verify
+    if (precision == BLPG_FP32) {
+      // FP64 re-solve of every LP whose FP32 result failed the check; the LP count is
+      // read on the device (q[21]), so no host synchronisation is needed here.
+      Bin fb;
+      fb.kernel = (size_t)(B->mmax + 2) * (B->nmax + 2) * 8 + 16384 <= ctx->tuner.lds_limit()
+                      ? BLPG_KERNEL_LDS
+                      : BLPG_KERNEL_STREAM;
+      fb.nt = 256;
+      fb.rcap = B->mmax + 2;
+      fb.ccap = B->nmax + 2;
+      StreamLaunch L{};
+      double e2d, e2p, t2, k2;
+      blpg_options o64 = ctx->opt;
+      o64.precision = BLPG_FP64;
+      resolve(o64, e2d, e2p, t2, k2);
+      L.precision = BLPG_FP64;
+      L.lds_tab = fb.kernel == BLPG_KERNEL_LDS;
+      L.nt = fb.nt;
+      L.desc = B->d_desc; L.pool = B->d_pool; L.spool = B->d_spool;
+      L.idx = B->d_fail; L.count = -1; L.count_dev = q + 21; L.queue = q + 22; L.out = B->out;
+      L.eps_d = e2d; L.eps_p = e2p; L.tol = t2;
+      L.max_iter = ctx->opt.max_iter; L.bland_after = bland_after(ctx->opt);
+      L.rcap = fb.rcap; L.ccap = fb.ccap;
+      const int per_cu = stream_max_blocks_per_cu(0, L.lds_tab, fb.nt, fb.rcap, fb.ccap);
+      L.grid = persistent_grid(ctx, per_cu, B->count);
+      if (!L.lds_tab) {
+        L.ws_stride = ((int64_t)fb.rcap * fb.ccap + 63) & ~63ll;
+        const int rc = grow(ctx, ctx->ws, 8 * (size_t)L.ws_stride * L.grid);
+        if (rc) return rc;
+        L.ws = ctx->ws.p;
+      }
+      HIPCHK(ctx, launch_fill_i32(q + 22, 0, 1, st));
+      HIPCHK(ctx, launch_stream(L, st));
+      // re-check the FP64 results
+      VerifyLaunch V2 = V;
+      V2.idx = B->d_fail;
+      V2.count = -1;
+      V2.count_dev = q + 21;
+      V2.fp32_source = 0;
+      V2.queue = q + 23;
+      V2.fail_list = nullptr;
+      V2.fail_count = nullptr;
+      HIPCHK(ctx, launch_fill_i32(q + 23, 0, 1, st));
+      HIPCHK(ctx, launch_verify(V2, st));
+    }
+    HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));  // end of fallback
+  }
+  return BLPG_OK;
+}
+
+int blpg_batch_sync(blpg_ctx* ctx) {
+  if (!ctx) return BLPG_E_INVALID;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return BLPG_OK;
+}
+
+int blpg_batch_info_get(blpg_ctx* ctx, const blpg_batch* B, blpg_batch_info* info) {
+  if (!ctx || !B || !info) return BLPG_E_INVALID;
+  *info = B->info;
+  const int nb = (int)B->bins.size();
+  float ms = 0;
+  for (int i = 0; i < nb; ++i) {
+    if (hipEventElapsedTime(&ms, B->ev[1 + 2 * i], B->ev[2 + 2 * i]) == hipSuccess)
+      info->kernel_ms[i] = ms;
+  }
+  const int e_solve = 1 + 2 * nb;
+  info->verify_ms = info->fallback_ms = 0;
+  if (B->nev > e_solve + 1 &&
+      hipEventElapsedTime(&ms, B->ev[e_solve], B->ev[e_solve + 1]) == hipSuccess)
+    info->verify_ms = ms;
+  if (B->nev > e_solve + 2 &&
+      hipEventElapsedTime(&ms, B->ev[e_solve + 1], B->ev[e_solve + 2]) == hipSuccess)
+    info->fallback_ms = ms;
+  if (B->nev > 0 && hipEventElapsedTime(&ms, B->ev[0], B->ev[B->nev - 1]) == hipSuccess)
+    info->total_ms = ms;
+  return BLPG_OK;
+}
+
+int blpg_batch_download(blpg_ctx* ctx, blpg_batch* B, blpg_result* out) {
+  if (!ctx || !B || (B->count > 0 && !out)) return BLPG_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int64_t n = B->count;
+  if (n == 0) return BLPG_OK;
+  std::vector<int32_t> i32(n * 6);
+  std::vector<double> f64(n * 5);
+  std::vector<double> x(B->x_elems * 2);
+  std::vector<int32_t> basis(B->basis_elems);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(i32.data(), B->out.status, sizeof(int32_t) * n * 6, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(f64.data(), B->out.obj, sizeof(double) * n * 5, hipMemcpyDeviceToHost));
+  if (B->x_elems)
+    HIPCHK(ctx, hipMemcpy(x.data(), B->out.x, sizeof(double) * B->x_elems * 2, hipMemcpyDeviceToHost));
+  if (B->basis_elems)
+    HIPCHK(ctx, hipMemcpy(basis.data(), B->out.basis, sizeof(int32_t) * B->basis_elems,
+                          hipMemcpyDeviceToHost));
+  const bool verified = ctx->opt.verify || B->precision == BLPG_FP32;
+  int64_t fbc = 0, piv = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    blpg_result& r = out[k];
+    const LpDesc& d = B->desc[k];
+    r.status = i32[k];
+    r.pivots = i32[n + k];
+    r.phase1_pivots = i32[2 * n + k];
+    const int ok = i32[4 * n + k];
+    r.fallback_used = i32[5 * n + k];
+    fbc += r.fallback_used;
+    piv += r.pivots;
+    const bool good = verified && ok == 1;
+    r.objective = good ? f64[2 * n + k] : f64[k];
+    r.kkt_primal = verified ? f64[3 * n + k] : -1.0;
+    r.kkt_dual = verified ? f64[4 * n + k] : -1.0;
+    if (B->precision == BLPG_FP32 && verified && ok == 0 && r.status == BLPG_OPTIMAL)
+      r.status = BLPG_NUMERIC;
+    if (r.x) {
+      const double* src = (good ? x.data() + B->x_elems : x.data()) + d.x;
+      std::memcpy(r.x, src, sizeof(double) * d.n);
+    }
+    if (r.basis) std::memcpy(r.basis, basis.data() + d.basis, sizeof(int32_t) * d.m);
+  }
+  B->info.fallback_count = fbc;
+  B->info.total_pivots = piv;
+  return BLPG_OK;
+}
+
+int blpg_solve_batch(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_result* out) {
+  if (!ctx) return BLPG_E_INVALID;
+  if (count == 0) return BLPG_OK;
+  blpg_batch* B = nullptr;
+  int rc = blpg_batch_upload(ctx, lps, count, &B);
+  if (rc) return rc;
+  rc = blpg_batch_solve(ctx, B);
+  if (!rc) rc = blpg_batch_sync(ctx);
+  if (!rc) rc = blpg_batch_download(ctx, B, out);
+  blpg_batch_free(ctx, B);
+  return rc;
+}
+
+int blpg_solve_shared_A(blpg_ctx* ctx, int32_t m, int32_t n, const double* A, const double* b,
+                        const int8_t* sense, const double* C, int64_t count, blpg_result* out) {
+  if (!ctx || !A || !b || !C || count < 0 || m < 1 || n < 1)
+    return fail(ctx, BLPG_E_INVALID, "invalid arguments to blpg_solve_shared_A");
+  std::vector<blpg_lp> lps(count);
+  for (int64_t k = 0; k < count; ++k) {
+    lps[k].m = m;
+    lps[k].n = n;
+    lps[k].A = A;
+    lps[k].b = b;
+    lps[k].c = C + k * (int64_t)n;
+    lps[k].sense = sense;
+    lps[k].family = -1;
+  }
+  return blpg_solve_batch(ctx, lps.data(), count, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// kernels.hpp -- host-side launch interface of the HIP kernels (no device code).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "blpg_internal.hpp"
+
+namespace blpg {
+
+// Size-generic simplex kernel (kernel_stream.hip): tableau in LDS or in an HBM slab.
+struct StreamLaunch {
+  int precision;  // 0 fp64, 1 fp32
+  bool lds_tab;
+  int nt;         // threads per workgroup: 64, 128, 256, 512
+  const LpDesc* desc;
+  const double* pool;
+  const int8_t* spool;
+  const int32_t* idx;  // LP ids of this bin
+  int32_t count;
+  const int32_t* count_dev = nullptr;  // if set, the LP count is read on the device
+  int32_t* queue;      // work-queue counter (must be 0 at launch)
+  Out out;
+  double eps_d, eps_p, tol;
+  int32_t max_iter, bland_after;
+  void* ws;            // HBM slabs (lds_tab == false): grid * ws_stride elements
+  int64_t ws_stride;
+  int rcap, ccap;      // largest m+2, n+2 of the bin
+  int grid;
+};
+size_t stream_lds_bytes(int precision, bool lds_tab, int nt, int rcap, int ccap);
+int stream_max_blocks_per_cu(int precision, bool lds_tab, int nt, int rcap, int ccap);
+hipError_t launch_stream(const StreamLaunch& L, hipStream_t st);
+
+// Register-tiled simplex kernel (kernel_tile.hip): tableau resident in VGPRs.
+struct TileLaunch {
+  int precision;
+  int variant;  // index into the tile variant table
+  const LpDesc* desc;
+  const double* pool;
+  const int8_t* spool;
+  const int32_t* idx;
+  int32_t count;
+  const int32_t* count_dev = nullptr;
+  int32_t* queue;
+  Out out;
+  double eps_d, eps_p, tol;
+  int32_t max_iter, bland_after;
+  int grid;
+};
+struct TileVariant {
+  int nw, tr, rpt, cpt;  // waves per LP, row groups, rows / columns per thread
+  int rcap() const { return tr * rpt; }
+  int ccap() const { return (nw * 64 / tr) * cpt; }
+};
+int tile_variant_count(int precision);
+TileVariant tile_variant(int precision, int v);
+int tile_max_blocks_per_cu(int precision, int v);
+hipError_t launch_tile(const TileLaunch& L, hipStream_t st);
+
+// FP64 check of final bases (kernel_verify.hip).
+struct VerifyLaunch {
+  const LpDesc* desc;
+  const double* pool;
+  const int8_t* spool;
+  const int32_t* idx;  // LPs to check (nullptr: all)
+  int32_t count;
+  const int32_t* count_dev = nullptr;
+  Out out;             // status / basis / unb read from here
+  Check chk;
+  double tol;          // relative KKT tolerance
+  int fp32_source;     // 1: results came from the FP32 tableau
+  double* ws;          // grid * ws_stride doubles (LU factors when they do not fit LDS)
+  int64_t ws_stride;
+  int mcap, ncap;      // largest m, n
+  int grid;
+  int32_t* queue;
+  int32_t* fail_list;  // LP ids that failed the check (appended)
+  int32_t* fail_count;
+};
+hipError_t launch_verify(const VerifyLaunch& L, hipStream_t st);
+int verify_max_blocks_per_cu(int mcap, int ncap, bool lds);
+size_t verify_lds_bytes(int mcap, int ncap, bool lds);
+
+// small helpers (kernel_util.hip)
+hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
+
+}  // namespace blpg

@@ -1,0 +1,111 @@
+// tuner.hpp -- per-size-bin kernel routing.
+//
+// For each LP shape the tuner picks one of:
+//   TILE   the register-resident kernel (smallest tile variant whose capacity covers
+//          (m+2) x (n+2) and whose registers fit),
+//   LDS    the streamed kernel with the tableau in LDS (when it fits the CU's LDS),
+//   STREAM the streamed kernel with the tableau in an HBM slab (any size).
+// All three perform identical arithmetic, so the choice changes speed, never results.
+// A measured preference table (set_preference, filled by the benchmark's autotune
+// pass and optionally loaded from a cache file) overrides the static order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <utility>
+
+#include "../../include/blpg/blpg.h"
+#include "kernels.hpp"
+
+namespace blpg {
+
+struct Route {
+  int kernel;   // BLPG_KERNEL_TILE / LDS / STREAM
+  int variant;  // tile variant
+  int nt;       // stream threads per workgroup
+  int rcap;
+  int bin;      // shape class (distinct bins for distinct shapes)
+};
+
+class Tuner {
+ public:
+  void init(const hipDeviceProp_t& p) {
+    lds_ = p.sharedMemPerBlock > 0 ? (size_t)p.sharedMemPerBlock : 65536;
+  }
+  size_t lds_limit() const { return lds_; }
+
+  static int stream_threads(int m, int n) {
+    const int work = (m + 2) * (n + 2);
+    if (work <= 64 * 40) return 64;
+    if (work <= 128 * 48) return 128;
+    return 256;
+  }
+
+  // preference key: (precision, m, n) -> (kernel, variant or nt)
+  void set_preference(int precision, int m, int n, int kernel, int param) {
+    pref_[key(precision, m, n)] = {kernel, param};
+  }
+  void clear_preferences() { pref_.clear(); }
+
+  Route route(int m, int n, int precision, int forced) const {
+    Route r{};
+    r.bin = ((m + 15) / 16) * 1024 + (n + 15) / 16;
+    r.rcap = m + 2;
+    r.nt = stream_threads(m, n);
+    const int esz = precision == BLPG_FP64 ? 8 : 4;
+    const int tv = best_tile(m, n, precision);
+    const size_t lds_need =
+        stream_lds_bytes(precision, true, r.nt, m + 2, n + 2);
+    const bool lds_ok = lds_need <= lds_ && stream_max_blocks_per_cu(precision, true, r.nt, m + 2, n + 2) > 0;
+    (void)esz;
+    auto it = pref_.find(key(precision, m, n));
+    if (forced == BLPG_KERNEL_AUTO && it != pref_.end()) {
+      const int k = it->second.first;
+      if (k == BLPG_KERNEL_TILE && it->second.second >= 0 &&
+          it->second.second < tile_variant_count(precision) &&
+          fits_tile(it->second.second, m, n, precision)) {
+        r.kernel = k; r.variant = it->second.second; return r;
+      }
+      if (k == BLPG_KERNEL_LDS && lds_ok) { r.kernel = k; if (it->second.second) r.nt = it->second.second; return r; }
+      if (k == BLPG_KERNEL_STREAM) { r.kernel = k; if (it->second.second) r.nt = it->second.second; return r; }
+    }
+    if ((forced == BLPG_KERNEL_AUTO || forced == BLPG_KERNEL_TILE) && tv >= 0) {
+      r.kernel = BLPG_KERNEL_TILE;
+      r.variant = tv;
+      return r;
+    }
+    if ((forced == BLPG_KERNEL_AUTO || forced == BLPG_KERNEL_LDS || forced == BLPG_KERNEL_TILE) &&
+        lds_ok) {
+      r.kernel = BLPG_KERNEL_LDS;
+      return r;
+    }
+    r.kernel = BLPG_KERNEL_STREAM;
+    return r;
+  }
+
+  static bool fits_tile(int v, int m, int n, int precision) {
+    const TileVariant t = tile_variant(precision, v);
+    return t.rcap() >= m + 2 && t.ccap() >= n + 2;
+  }
+
+  // smallest-capacity tile variant that holds the LP
+  static int best_tile(int m, int n, int precision) {
+    int best = -1;
+    long best_cap = 0;
+    for (int v = 0; v < tile_variant_count(precision); ++v) {
+      if (!fits_tile(v, m, n, precision)) continue;
+      const TileVariant t = tile_variant(precision, v);
+      const long cap = (long)t.rcap() * t.ccap();
+      if (best < 0 || cap < best_cap) { best = v; best_cap = cap; }
+    }
+    return best;
+  }
+
+ private:
+  static long key(int p, int m, int n) { return ((long)p << 40) | ((long)m << 20) | n; }
+  size_t lds_ = 65536;
+  std::map<long, std::pair<int, int>> pref_;
+};
+
+}  // namespace blpg
