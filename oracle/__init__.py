@@ -46,6 +46,8 @@ def lib() -> C.CDLL:
         P = C.POINTER
         for nm in ("orc_solve_d", "orc_solve_f"):
             getattr(L, nm).argtypes = [P(_LP), P(_Opts), P(_Res)]
+        for nm in ("orc_solve_warm_d", "orc_solve_warm_f"):
+            getattr(L, nm).argtypes = [P(_LP), P(_Opts), P(C.c_int32), P(_Res)]
         L.orc_solve_tab_d.argtypes = [P(_LP), P(_Opts), P(_Res), P(C.c_double)]
         L.orc_solve_tab_f.argtypes = [P(_LP), P(_Opts), P(_Res), P(C.c_float)]
         L.orc_ref_rule_solve.argtypes = [C.c_int32, C.c_int32, P(C.c_double), P(C.c_double),
@@ -87,9 +89,10 @@ def defaults(precision: str, m: int, n: int) -> dict:
     return d
 
 
-def solve(A, b, c, sense=None, precision: str = "fp64", tab: bool = False, **kw) -> dict:
+def solve(A, b, c, sense=None, precision: str = "fp64", tab: bool = False, warm=None,
+          **kw) -> dict:
     """Replay the build's pivot rule on the CPU.  Keyword overrides: eps_d, eps_p,
-    tol_feas, max_iter, bland_after."""
+    tol_feas, max_iter, bland_after.  `warm`: target basis (m variable ids) to crash in."""
     L, keep = _mk(A, b, c, sense)
     m, n = keep[0].shape
     d = defaults(precision, m, n)
@@ -101,7 +104,11 @@ def solve(A, b, c, sense=None, precision: str = "fp64", tab: bool = False, **kw)
     r.x = _dp(x)
     r.basis = basis.ctypes.data_as(C.POINTER(C.c_int32))
     out = {}
-    if tab:
+    if warm is not None:
+        w = np.ascontiguousarray(warm, dtype=np.int32)
+        fn = lib().orc_solve_warm_f if precision == "fp32" else lib().orc_solve_warm_d
+        rc = fn(C.byref(L), C.byref(o), w.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(r))
+    elif tab:
         dt = np.float32 if precision == "fp32" else np.float64
         T = np.zeros((m + 2, n + 2), dtype=dt)
         fn = lib().orc_solve_tab_f if precision == "fp32" else lib().orc_solve_tab_d

@@ -70,6 +70,12 @@ int orc_solve_f(const orc_lp *lp, const orc_opts *o, orc_res *res);
 int orc_solve_tab_d(const orc_lp *lp, const orc_opts *o, orc_res *res, double *tab_out);
 int orc_solve_tab_f(const orc_lp *lp, const orc_opts *o, orc_res *res, float *tab_out);
 
+/* Warm start: `warm` (m variable ids, -1 = none) is a target basis whose structurals are
+ * pivoted in first (crash: largest |T[i][s]| among rows whose basic variable is not a
+ * target; skipped when none exceeds eps_p), then phase I / II repair and finish. */
+int orc_solve_warm_d(const orc_lp *lp, const orc_opts *o, const int32_t *warm, orc_res *res);
+int orc_solve_warm_f(const orc_lp *lp, const orc_opts *o, const int32_t *warm, orc_res *res);
+
 /* Batch driver for the CPU baseline: `count` LPs laid out as in the packed pool
  * (offsets in doubles / int8 entries), solved with `threads` OpenMP threads. */
 int orc_solve_batch_d(int64_t count, const int32_t *m, const int32_t *n, const int64_t *a_off,

@@ -25,6 +25,7 @@ ARCH = os.environ.get("BLPG_ARCH", "gfx950")
 #   the CPU oracle operation for operation.
 # packed-fp32-ops disabled: no v_pk_* FP32 instructions are generated.
 HIPFLAGS = [
+    "-mllvm", "-simplifycfg-sink-common=false",
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-fno-gpu-flush-denormals-to-zero", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -34,9 +34,13 @@ struct blpg_ctx {
   hipStream_t stream = nullptr;
   hipDeviceProp_t props{};
   std::string err;
-  DevBuf ws;   // HBM slabs of the streamed kernel
+  DevBuf ws;   // HBM slabs of the streamed kernel (one region per concurrent bin)
+  DevBuf ws2;  // HBM slabs of the fallback kernel
   DevBuf vws;  // LU workspace of the verify kernel
   Tuner tuner;
+  static constexpr int kAux = 4;  // size bins run concurrently on these (default priority)
+  hipStream_t aux[kAux] = {};
+  hipEvent_t fork = nullptr;
 };
 
 namespace {
@@ -161,6 +165,15 @@ int blpg_create(blpg_ctx** out, const blpg_options* opt) {
     return BLPG_E_HIP;
   }
   ctx->tuner.init(ctx->props);
+  for (int i = 0; i < blpg_ctx::kAux; ++i)
+    if (hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking) != hipSuccess) {
+      blpg_destroy(ctx);
+      return BLPG_E_HIP;
+    }
+  if (hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess) {
+    blpg_destroy(ctx);
+    return BLPG_E_HIP;
+  }
   *out = ctx;
   return BLPG_OK;
 }
@@ -169,7 +182,11 @@ void blpg_destroy(blpg_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) { (void)hipStreamSynchronize(ctx->stream); (void)hipStreamDestroy(ctx->stream); }
+  for (int i = 0; i < blpg_ctx::kAux; ++i)
+    if (ctx->aux[i]) { (void)hipStreamSynchronize(ctx->aux[i]); (void)hipStreamDestroy(ctx->aux[i]); }
+  if (ctx->fork) (void)hipEventDestroy(ctx->fork);
   if (ctx->ws.p) (void)hipFree(ctx->ws.p);
+  if (ctx->ws2.p) (void)hipFree(ctx->ws2.p);
   if (ctx->vws.p) (void)hipFree(ctx->vws.p);
   delete ctx;
 }
@@ -398,48 +415,69 @@ void blpg_batch_free(blpg_ctx* ctx, blpg_batch* B) {
   delete B;
 }
 
-static int launch_bin(blpg_ctx* ctx, blpg_batch* B, const Bin& b, int precision,
-                      const int32_t* idx, int32_t count, int32_t* queue, hipStream_t st) {
+// Launch plan of one size bin: kernel, persistent grid and HBM slab size.
+struct BinPlan {
+  int grid = 1;
+  size_t ws_bytes = 0;
+  int64_t ws_stride = 0;
+};
+
+static int plan_bin(blpg_ctx* ctx, const Bin& b, int precision, int64_t count, BinPlan& P) {
+  if (b.kernel == BLPG_KERNEL_TILE) {
+    const int per_cu = tile_max_blocks_per_cu(precision, b.variant);
+    if (per_cu <= 0) return fail(ctx, BLPG_E_INVALID, "tile variant %d cannot be resident", b.variant);
+    P.grid = persistent_grid(ctx, per_cu, count);
+    return BLPG_OK;
+  }
+  const bool lds = b.kernel == BLPG_KERNEL_LDS;
+  const int per_cu = stream_max_blocks_per_cu(precision, lds, b.nt, b.rcap, b.ccap);
+  if (per_cu <= 0)
+    return fail(ctx, BLPG_E_INVALID, "kernel cannot be resident (rcap %d ccap %d lds %d)",
+                b.rcap, b.ccap, (int)lds);
+  P.grid = persistent_grid(ctx, per_cu, count);
+  if (!lds) {
+    const size_t esz = precision == BLPG_FP64 ? 8 : 4;
+    P.ws_stride = ((int64_t)b.rcap * b.ccap + 63) & ~63ll;
+    P.ws_bytes = (esz * (size_t)P.ws_stride * P.grid + 255) & ~(size_t)255;
+  }
+  return BLPG_OK;
+}
+
+static int launch_bin(blpg_ctx* ctx, blpg_batch* B, const Bin& b, const BinPlan& P,
+                      int precision, const int32_t* idx, int32_t count,
+                      const int32_t* count_dev, int32_t* queue, void* ws,
+                      const int32_t* warm, hipStream_t st) {
   double eps_d, eps_p, tol, kkt;
   blpg_options o = ctx->opt;
   o.precision = precision;
   resolve(o, eps_d, eps_p, tol, kkt);
+  HIPCHK(ctx, launch_fill_i32(queue, 0, 1, st));
   if (b.kernel == BLPG_KERNEL_TILE) {
     TileLaunch L{};
     L.precision = precision;
     L.variant = b.variant;
     L.desc = B->d_desc; L.pool = B->d_pool; L.spool = B->d_spool;
-    L.idx = idx; L.count = count; L.queue = queue; L.out = B->out;
+    L.idx = idx; L.count = count; L.count_dev = count_dev; L.queue = queue; L.out = B->out;
     L.eps_d = eps_d; L.eps_p = eps_p; L.tol = tol;
     L.max_iter = o.max_iter; L.bland_after = bland_after(o);
-    L.grid = persistent_grid(ctx, tile_max_blocks_per_cu(precision, b.variant), count);
-    HIPCHK(ctx, launch_fill_i32(queue, 0, 1, st));
+    L.grid = P.grid;
+    L.warm = warm;
     HIPCHK(ctx, launch_tile(L, st));
     return BLPG_OK;
   }
-  const bool lds = b.kernel == BLPG_KERNEL_LDS;
   StreamLaunch L{};
   L.precision = precision;
-  L.lds_tab = lds;
+  L.lds_tab = b.kernel == BLPG_KERNEL_LDS;
   L.nt = b.nt;
   L.desc = B->d_desc; L.pool = B->d_pool; L.spool = B->d_spool;
-  L.idx = idx; L.count = count; L.queue = queue; L.out = B->out;
+  L.idx = idx; L.count = count; L.count_dev = count_dev; L.queue = queue; L.out = B->out;
   L.eps_d = eps_d; L.eps_p = eps_p; L.tol = tol;
   L.max_iter = o.max_iter; L.bland_after = bland_after(o);
   L.rcap = b.rcap; L.ccap = b.ccap;
-  const int per_cu = stream_max_blocks_per_cu(precision, lds, b.nt, b.rcap, b.ccap);
-  if (per_cu <= 0)
-    return fail(ctx, BLPG_E_INVALID, "kernel cannot be resident (rcap %d ccap %d lds %d)",
-                b.rcap, b.ccap, (int)lds);
-  L.grid = persistent_grid(ctx, per_cu, count);
-  if (!lds) {
-    const size_t esz = precision == BLPG_FP64 ? 8 : 4;
-    L.ws_stride = ((int64_t)b.rcap * b.ccap + 63) & ~63ll;
-    const int rc = grow(ctx, ctx->ws, esz * (size_t)L.ws_stride * L.grid);
-    if (rc) return rc;
-    L.ws = ctx->ws.p;
-  }
-  HIPCHK(ctx, launch_fill_i32(queue, 0, 1, st));
+  L.grid = P.grid;
+  L.ws = ws;
+  L.ws_stride = P.ws_stride;
+  L.warm = warm;
   HIPCHK(ctx, launch_stream(L, st));
   return BLPG_OK;
 }
@@ -451,20 +489,42 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
   hipStream_t st = ctx->stream;
   const int precision = ctx->opt.precision;
   B->precision = precision;
-  B->nev = 0;
   int32_t* q = B->d_counters;
-  HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));
-  HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
-  for (size_t i = 0; i < B->bins.size(); ++i) {
-    const Bin& b = B->bins[i];
-    HIPCHK(ctx, hipEventRecord(B->ev[1 + 2 * i], st));
-    const int rc = launch_bin(ctx, B, b, precision, b.d_idx, (int32_t)b.ids.size(), q + i, st);
+  const int nb = (int)B->bins.size();
+
+  // ---- plans and one HBM workspace carved per bin ----
+  std::vector<BinPlan> plans(nb);
+  std::vector<size_t> ws_off(nb, 0);
+  size_t ws_total = 0;
+  for (int i = 0; i < nb; ++i) {
+    const int rc = plan_bin(ctx, B->bins[i], precision, (int64_t)B->bins[i].ids.size(), plans[i]);
     if (rc) return rc;
-    HIPCHK(ctx, hipEventRecord(B->ev[2 + 2 * i], st));
+    ws_off[i] = ws_total;
+    ws_total += plans[i].ws_bytes;
   }
-  B->nev = 1 + 2 * (int)B->bins.size();
-  HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));  // end of solve kernels
-  // FP64 check of the final bases, and the FP64 fallback for FP32 failures
+  if (ws_total) {
+    const int rc = grow(ctx, ctx->ws, ws_total);
+    if (rc) return rc;
+  }
+
+  HIPCHK(ctx, hipEventRecord(B->ev[0], st));
+  HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
+  // ---- fork: every size bin on its own stream, concurrently ----
+  HIPCHK(ctx, hipEventRecord(ctx->fork, st));
+  for (int i = 0; i < nb; ++i) {
+    const Bin& b = B->bins[i];
+    hipStream_t s = nb == 1 ? st : ctx->aux[i % blpg_ctx::kAux];
+    if (s != st) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->fork, 0));
+    HIPCHK(ctx, hipEventRecord(B->ev[1 + 2 * i], s));
+    const int rc = launch_bin(ctx, B, b, plans[i], precision, b.d_idx, (int32_t)b.ids.size(),
+                              nullptr, q + i, (char*)ctx->ws.p + ws_off[i], nullptr, s);
+    if (rc) return rc;
+    HIPCHK(ctx, hipEventRecord(B->ev[2 + 2 * i], s));
+    if (s != st) HIPCHK(ctx, hipStreamWaitEvent(st, B->ev[2 + 2 * i], 0));
+  }
+  B->nev = 1 + 2 * nb;
+  HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));  // all solve kernels done
+  // ---- FP64 check of the final bases, and the FP64 fallback for FP32 failures ----
   double eps_d, eps_p, tol, kkt;
   resolve(ctx->opt, eps_d, eps_p, tol, kkt);
   const bool do_verify = ctx->opt.verify || precision == BLPG_FP32;
@@ -476,10 +536,10 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
     V.fp32_source = precision == BLPG_FP32;
     V.mcap = B->mmax;
     V.ncap = B->nmax;
-    const bool lds = verify_lds_bytes(B->mmax, B->nmax, true) <= 100 * 1024;
-    V.grid = persistent_grid(ctx, verify_max_blocks_per_cu(B->mmax, B->nmax, lds), B->count);
-    V.ws_stride = ((int64_t)B->mmax * B->mmax + 63) & ~63ll;
-    if (!lds) {
+    V.grid = persistent_grid(ctx, verify_max_blocks_per_cu(B->mmax, B->nmax, true), B->count);
+    const int64_t kcap = std::min(B->mmax, B->nmax);
+    V.ws_stride = (kcap * kcap + 63) & ~63ll;
+    {
       const int rc = grow(ctx, ctx->vws, sizeof(double) * (size_t)V.ws_stride * V.grid);
       if (rc) return rc;
       V.ws = (double*)ctx->vws.p;
@@ -491,38 +551,25 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
     HIPCHK(ctx, launch_verify(V, st));
     HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));  // end of verify
     if (precision == BLPG_FP32) {
-      // FP64 re-solve of every LP whose FP32 result failed the check; the LP count is
-      // read on the device (q[21]), so no host synchronisation is needed here.
+      // FP64 re-solve of every LP whose FP32 result failed the check, warm-started from
+      // its FP32 basis (crash); the LP count is read on the device (q[21]), so no host
+      // synchronisation is needed.
       Bin fb;
-      fb.kernel = (size_t)(B->mmax + 2) * (B->nmax + 2) * 8 + 16384 <= ctx->tuner.lds_limit()
-                      ? BLPG_KERNEL_LDS
-                      : BLPG_KERNEL_STREAM;
-      fb.nt = 256;
       fb.rcap = B->mmax + 2;
       fb.ccap = B->nmax + 2;
-      StreamLaunch L{};
-      double e2d, e2p, t2, k2;
-      blpg_options o64 = ctx->opt;
-      o64.precision = BLPG_FP64;
-      resolve(o64, e2d, e2p, t2, k2);
-      L.precision = BLPG_FP64;
-      L.lds_tab = fb.kernel == BLPG_KERNEL_LDS;
-      L.nt = fb.nt;
-      L.desc = B->d_desc; L.pool = B->d_pool; L.spool = B->d_spool;
-      L.idx = B->d_fail; L.count = -1; L.count_dev = q + 21; L.queue = q + 22; L.out = B->out;
-      L.eps_d = e2d; L.eps_p = e2p; L.tol = t2;
-      L.max_iter = ctx->opt.max_iter; L.bland_after = bland_after(ctx->opt);
-      L.rcap = fb.rcap; L.ccap = fb.ccap;
-      const int per_cu = stream_max_blocks_per_cu(0, L.lds_tab, fb.nt, fb.rcap, fb.ccap);
-      L.grid = persistent_grid(ctx, per_cu, B->count);
-      if (!L.lds_tab) {
-        L.ws_stride = ((int64_t)fb.rcap * fb.ccap + 63) & ~63ll;
-        const int rc = grow(ctx, ctx->ws, 8 * (size_t)L.ws_stride * L.grid);
-        if (rc) return rc;
-        L.ws = ctx->ws.p;
-      }
-      HIPCHK(ctx, launch_fill_i32(q + 22, 0, 1, st));
-      HIPCHK(ctx, launch_stream(L, st));
+      fb.nt = 256;
+      const int tv = Tuner::best_tile(B->mmax, B->nmax, BLPG_FP64);
+      if (tv >= 0) { fb.kernel = BLPG_KERNEL_TILE; fb.variant = tv; }
+      else if (stream_lds_bytes(BLPG_FP64, true, fb.nt, fb.rcap, fb.ccap) <= ctx->tuner.lds_limit())
+        fb.kernel = BLPG_KERNEL_LDS;
+      else fb.kernel = BLPG_KERNEL_STREAM;
+      BinPlan FP;
+      int rc = plan_bin(ctx, fb, BLPG_FP64, B->count, FP);
+      if (rc) return rc;
+      if (FP.ws_bytes) { rc = grow(ctx, ctx->ws2, FP.ws_bytes); if (rc) return rc; }
+      rc = launch_bin(ctx, B, fb, FP, BLPG_FP64, B->d_fail, -1, q + 21, q + 22, ctx->ws2.p,
+                      B->out.basis, st);
+      if (rc) return rc;
       // re-check the FP64 results
       VerifyLaunch V2 = V;
       V2.idx = B->d_fail;

@@ -30,11 +30,12 @@ struct StreamTab {
   int* nb;
   uint8_t* dead;
   uint8_t* redund;
+  uint8_t* target;  // target[var] = 1: var belongs to the warm-start basis
   Reducer<T, NW> red;
 
   __device__ __forceinline__ T& at(int i, int j) { return tab[(int64_t)i * ld + j]; }
 
-  __device__ void init(const double* A, const double* b, const double* c, const int8_t* sense) {
+  __device__ __forceinline__ void init(const double* A, const double* b, const double* c, const int8_t* sense) {
     const int tid = threadIdx.x;
     for (int e = tid; e < R * C; e += NT) {
       const int i = e / C, j = e - (e / C) * C;
@@ -48,7 +49,39 @@ struct StreamTab {
     __syncthreads();
   }
 
-  __device__ int pricing(int orow, bool bland, T eps_d) {
+  __device__ __forceinline__ void set_target(const int32_t* warm) {
+    for (int v = threadIdx.x; v < n + m; v += NT) target[v] = 0;
+    __syncthreads();
+    if (warm)
+      for (int i = threadIdx.x; i < m; i += NT) {
+        const int v = warm[i];
+        if (v >= 0 && v < n + m) target[v] = 1;
+      }
+    __syncthreads();
+  }
+
+  // row of largest |T[i][s]| among constraint rows whose basic variable is not a target
+  __device__ __forceinline__ int absmax_row(int s, T eps_p) {
+    __syncthreads();
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    for (int i = 2 + (int)threadIdx.x; i < R; i += NT) {
+      const int v = bs[i];
+      if (redund[i] || v < 0 || target[v]) continue;
+      const T key = -tabs(at(i, s));
+      if (better(key, i, bv, bk)) { bv = key; bk = i; bp = i; }
+    }
+    red.argmin(bv, bk, bp);
+    if (bp < 0 || !(-bv > eps_p)) return -1;
+    return bp;
+  }
+
+  __device__ __forceinline__ int bs_of(int i) {
+    __syncthreads();
+    return uniform(bs[i]);
+  }
+
+  __device__ __forceinline__ int pricing(int orow, bool bland, T eps_d) {
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
     for (int j = 1 + (int)threadIdx.x; j < C; j += NT) {
@@ -62,7 +95,7 @@ struct StreamTab {
     return bp;
   }
 
-  __device__ int ratio(int s, bool bland, T eps_p, T& theta) {
+  __device__ __forceinline__ int ratio(int s, bool bland, T eps_p, T& theta) {
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
     for (int i = threadIdx.x; i < R; i += NT) {
@@ -80,14 +113,14 @@ struct StreamTab {
     return bp;
   }
 
-  __device__ void pivot(int r, int s) {
+  __device__ __forceinline__ void pivot(int r, int s) {
     const int tid = threadIdx.x;
     // every row's pivot-column entry (the ratio pass filled colbuf; refresh it here so
     // phase-I pivots chosen without a ratio pass see it too)
     __syncthreads();
     for (int i = tid; i < R; i += NT) colbuf[i] = at(i, s);
     __syncthreads();
-    const T p = colbuf[r];
+    const T p = uniform(colbuf[r]);
     const T rp = T(1) / p;
     for (int j = tid; j < C; j += NT) rowbuf[j] = (j == s) ? rp : at(r, j) * rp;
     if (tid == 0) { const int t = bs[r]; bs[r] = nb[s]; nb[s] = t; }
@@ -108,7 +141,7 @@ struct StreamTab {
     __syncthreads();
   }
 
-  __device__ int absmax_col(int r, T eps_p) {
+  __device__ __forceinline__ int absmax_col(int r, T eps_p) {
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
     for (int j = 1 + (int)threadIdx.x; j < C; j += NT) {
@@ -121,31 +154,31 @@ struct StreamTab {
     return bp;
   }
 
-  __device__ void kill_col(int j) {
+  __device__ __forceinline__ void kill_col(int j) {
     __syncthreads();
     if (threadIdx.x == 0) dead[j] = 1;
     for (int i = threadIdx.x; i < R; i += NT) at(i, j) = T(0);
     __syncthreads();
   }
 
-  __device__ T get(int i, int j) {
+  __device__ __forceinline__ T get(int i, int j) {
     __syncthreads();
-    return at(i, j);
+    return uniform(at(i, j));
   }
 
-  __device__ void set(int i, int j, T v) {
+  __device__ __forceinline__ void set(int i, int j, T v) {
     __syncthreads();
     if (threadIdx.x == 0) at(i, j) = v;
     __syncthreads();
   }
 
-  __device__ void mark_redund(int r) {
+  __device__ __forceinline__ void mark_redund(int r) {
     __syncthreads();
     if (threadIdx.x == 0) { redund[r] = 1; at(r, 0) = T(0); }
     __syncthreads();
   }
 
-  __device__ int min_rhs_row() {
+  __device__ __forceinline__ int min_rhs_row() {
     __syncthreads();
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
@@ -158,7 +191,7 @@ struct StreamTab {
     return bp;
   }
 
-  __device__ void setup_x0() {
+  __device__ __forceinline__ void setup_x0() {
     __syncthreads();
     const int x0 = C - 1;
     if (threadIdx.x == 0) dead[x0] = 0;
@@ -168,7 +201,7 @@ struct StreamTab {
     __syncthreads();
   }
 
-  __device__ int find_basic(int var) {
+  __device__ __forceinline__ int find_basic(int var) {
     __syncthreads();
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
@@ -178,7 +211,7 @@ struct StreamTab {
     return bp;
   }
 
-  __device__ int find_nonbasic(int var) {
+  __device__ __forceinline__ int find_nonbasic(int var) {
     __syncthreads();
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
@@ -188,12 +221,12 @@ struct StreamTab {
     return bp;
   }
 
-  __device__ int nb_of(int j) {
+  __device__ __forceinline__ int nb_of(int j) {
     __syncthreads();
-    return nb[j];
+    return uniform(nb[j]);
   }
 
-  __device__ void write_outputs(int lp, const LpDesc& d, const Out& out, int status, int it,
+  __device__ __forceinline__ void write_outputs(int lp, const LpDesc& d, const Out& out, int status, int it,
                                 int p1, int unb, T wfin) {
     __syncthreads();
     const int tid = threadIdx.x;
@@ -217,10 +250,11 @@ struct StreamTab {
   }
 };
 
-// LDS carve-up shared by host and device: [tab?][colbuf][rowbuf][bs][nb][dead][redund][slots]
+// LDS carve-up shared by host and device:
+// [tab?][colbuf][rowbuf][bs][nb][dead][redund][target][slots]
 template <typename T>
 struct StreamLayout {
-  size_t tab, colbuf, rowbuf, bs, nb, dead, redund, slots, total;
+  size_t tab, colbuf, rowbuf, bs, nb, dead, redund, target, slots, total;
   __host__ __device__ StreamLayout(bool lds_tab, int nw, int rcap, int ccap) {
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
     size_t o = 0;
@@ -231,6 +265,7 @@ struct StreamLayout {
     nb = o; o += al(sizeof(int) * ccap);
     dead = o; o += al(ccap);
     redund = o; o += al(rcap);
+    target = o; o += al(rcap + ccap);
     slots = o; o += al(sizeof(RedSlot<T>) * 2 * nw);
     total = o;
   }
@@ -243,7 +278,8 @@ __global__ __launch_bounds__(NT) void k_stream(const LpDesc* __restrict__ desc,
                                                const int32_t* __restrict__ idx, int32_t count,
                                                const int32_t* count_dev, int32_t* queue,
                                                Out out, Opt<T> o, T* ws,
-                                               int64_t ws_stride, int rcap, int ccap) {
+                                               int64_t ws_stride, int rcap, int ccap,
+                                               const int32_t* __restrict__ warm_pool) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_q;
   const StreamLayout<T> L(LDS_TAB, NT / 64, rcap, ccap);
@@ -254,6 +290,7 @@ __global__ __launch_bounds__(NT) void k_stream(const LpDesc* __restrict__ desc,
   t.nb = (int*)(smem + L.nb);
   t.dead = smem + L.dead;
   t.redund = smem + L.redund;
+  t.target = smem + L.target;
   t.red.init((RedSlot<T>*)(smem + L.slots));
   t.tab = LDS_TAB ? (T*)(smem + L.tab) : ws + (int64_t)blockIdx.x * ws_stride;
   if (count_dev) count = *count_dev;
@@ -261,16 +298,18 @@ __global__ __launch_bounds__(NT) void k_stream(const LpDesc* __restrict__ desc,
     __syncthreads();
     if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
     __syncthreads();
-    const int q = s_q;
+    const int q = uniform(s_q);
     if (q >= count) break;
     const int lp = idx ? idx[q] : q;
     const LpDesc d = desc[lp];
     t.m = d.m; t.n = d.n; t.R = d.m + 2; t.C = d.n + 2; t.ld = t.C;
     const int8_t* sense = d.s >= 0 ? spool + d.s : nullptr;
     t.init(pool + d.a, pool + d.b, pool + d.c, sense);
+    const int32_t* warm = warm_pool ? warm_pool + d.basis : nullptr;
+    t.set_target(warm);
     int status, it, p1, unb;
     T wfin;
-    run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin);
+    run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, warm);
     t.write_outputs(lp, d, out, status, it, p1, unb, wfin);
   }
 }
@@ -310,13 +349,13 @@ hipError_t launch_stream(const StreamLaunch& L, hipStream_t st) {
     Opt<double> o{L.eps_d, L.eps_p, L.tol, L.max_iter, L.bland_after};
     void* args[] = {(void*)&L.desc, (void*)&L.pool, (void*)&L.spool, (void*)&L.idx,
                     (void*)&L.count, (void*)&L.count_dev, (void*)&L.queue, (void*)&L.out, (void*)&o,
-                    (void*)&L.ws, (void*)&L.ws_stride, (void*)&L.rcap, (void*)&L.ccap};
+                    (void*)&L.ws, (void*)&L.ws_stride, (void*)&L.rcap, (void*)&L.ccap, (void*)&L.warm};
     return hipLaunchKernel(fn, dim3(L.grid), dim3(L.nt), args, lds, st);
   }
   Opt<float> o{(float)L.eps_d, (float)L.eps_p, (float)L.tol, L.max_iter, L.bland_after};
   void* args[] = {(void*)&L.desc, (void*)&L.pool, (void*)&L.spool, (void*)&L.idx,
                   (void*)&L.count, (void*)&L.count_dev, (void*)&L.queue, (void*)&L.out, (void*)&o,
-                  (void*)&L.ws, (void*)&L.ws_stride, (void*)&L.rcap, (void*)&L.ccap};
+                  (void*)&L.ws, (void*)&L.ws_stride, (void*)&L.rcap, (void*)&L.ccap, (void*)&L.warm};
   return hipLaunchKernel(fn, dim3(L.grid), dim3(L.nt), args, lds, st);
 }
 

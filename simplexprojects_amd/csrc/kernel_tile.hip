@@ -1,4 +1,21 @@
-// kernel_tile.hip -- register-resident simplex kernel (variants added below).
+// kernel_tile.hip -- register-resident simplex kernel for gfx950.
+//
+// One workgroup (NW wavefronts) per LP, persistent over a work queue.  The condensed
+// tableau never leaves the vector register file: thread t = gc + TC*gr owns rows
+// gr + TR*k (k < RPT) and columns gc + TC*l (l < CPT), i.e. an RPT x CPT block held in
+// `v[RPT][CPT]`.  A CU's 512 KB of VGPRs is 3.2x its LDS, so tableaux up to ~207x205 FP64
+// stay on-chip, and the rank-1 update is RPT*CPT register FMAs per thread with no memory
+// traffic at all.  Per pivot LDS carries only
+//   colbuf  the pivot column (written by the TR owners of column s, 16 B per row)
+//   rhsbuf  the right-hand side column (for the ratio test)
+//   rowbuf  the scaled pivot row (written by the TC owners of row r)
+// and the (value, index) reduction partials.  Indices of the pivot row / column are
+// wave-uniform, so their extraction from the register block is a scalar branch over the
+// unrolled k / l slots, not a dynamic register index.
+//
+// Arithmetic and rule are those of kernel_stream.hip and the oracle, operation for
+// operation: results are bit-identical across kernels.  Replaces the reference's
+// mykernel (simplex.cu:35-385), whose tableau lives in global memory.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -6,9 +23,470 @@
 
 namespace blpg {
 
-int tile_variant_count(int) { return 0; }
-TileVariant tile_variant(int, int) { return TileVariant{1, 8, 1, 1}; }
-int tile_max_blocks_per_cu(int, int) { return 0; }
-hipError_t launch_tile(const TileLaunch&, hipStream_t) { return hipErrorInvalidValue; }
+template <typename T, int NW, int TC, int RPT, int CPT>
+struct TileTab {
+  static constexpr int NT = NW * 64;
+  static constexpr int TR = NT / TC;
+  static constexpr int RCAP = TR * RPT;
+  static constexpr int CCAP = TC * CPT;
+  static_assert(TC <= 64 && 64 % TC == 0, "a row group must sit inside one wavefront");
+  static_assert(TR >= 2, "rows 0 and 1 must be in slot k = 0");
+  static_assert(CPT <= 32, "dead-column mask is 32 bits");
+
+  T v[RPT][CPT];
+  uint32_t dead;  // bit l: column gc + TC*l cannot enter
+  int gr, gc, tid;
+  int R, C, m, n;
+  int col_ready;  // column whose entries are in colbuf (-1: none)
+  T* colbuf;
+  T* rhsbuf;
+  T* rowbuf;
+  T* scal;
+  int* bs;
+  int* nb;
+  uint8_t* redund;
+  uint8_t* target;  // target[var] = 1: var belongs to the warm-start basis
+  Reducer<T, NW> red;
+
+  __device__ __forceinline__ int row_of(int k) const { return gr + TR * k; }
+  __device__ __forceinline__ int col_of(int l) const { return gc + TC * l; }
+
+  __device__ __forceinline__ void init(const double* A, const double* b, const double* c, const int8_t* sense) {
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const int i = row_of(k);
+#pragma unroll
+      for (int l = 0; l < CPT; ++l) {
+        const int j = col_of(l);
+        v[k][l] = (i < R && j < C) ? init_entry<T>(i, j, m, n, A, b, c, sense) : T(0);
+      }
+    }
+    dead = 0;
+#pragma unroll
+    for (int l = 0; l < CPT; ++l) {
+      const int j = col_of(l);
+      if (j == 0 || j >= C - 1) dead |= 1u << l;
+    }
+    for (int i = tid; i < RCAP; i += NT) { bs[i] = (i >= 2 && i < R) ? n + i - 2 : -1; redund[i] = 0; }
+    for (int j = tid; j < CCAP; j += NT) nb[j] = (j == 0 || j >= C) ? -1 : (j <= n ? j - 1 : n + m);
+    col_ready = -1;
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ void set_target(const int32_t* warm) {
+    for (int v = tid; v < RCAP + CCAP; v += NT) target[v] = 0;
+    __syncthreads();
+    if (warm)
+      for (int i = tid; i < m; i += NT) {
+        const int v = warm[i];
+        if (v >= 0 && v < n + m) target[v] = 1;
+      }
+    __syncthreads();
+  }
+
+  // row of largest |T[i][s]| among constraint rows whose basic variable is not a target
+  __device__ __forceinline__ int absmax_row(int s, T eps_p) {
+    __syncthreads();
+    publish_col(s);
+    __syncthreads();
+    col_ready = s;
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    for (int i = 2 + tid; i < R; i += NT) {
+      const int v = bs[i];
+      if (redund[i] || v < 0 || target[v]) continue;
+      const T key = -tabs(colbuf[i]);
+      if (better(key, i, bv, bk)) { bv = key; bk = i; bp = i; }
+    }
+    red.argmin(bv, bk, bp);
+    if (bp < 0 || !(-bv > eps_p)) return -1;
+    return bp;
+  }
+
+  __device__ __forceinline__ int bs_of(int i) {
+    __syncthreads();
+    return uniform(bs[i]);
+  }
+
+  // the TR owners of column s store it in colbuf
+  __device__ __forceinline__ void publish_col(int s) {
+    const int ls = s / TC, gcs = s - ls * TC;
+#pragma unroll
+    for (int l = 0; l < CPT; ++l) {
+      if (l == ls) {
+        if (gc == gcs) {
+#pragma unroll
+          for (int k = 0; k < RPT; ++k) colbuf[row_of(k)] = v[k][l];
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ int pricing(int orow, bool bland, T eps_d) {
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    if (gr == orow) {
+      if (bland) {
+#pragma unroll
+        for (int l = 0; l < CPT; ++l) {
+          const int j = col_of(l);
+          const T d = v[0][l];
+          if (!((dead >> l) & 1u) && d < -eps_d) {
+            const T key = T(nb[j]);
+            if (better(key, j, bv, bk)) { bv = key; bk = j; bp = j; }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int l = 0; l < CPT; ++l) {
+          const int j = col_of(l);
+          const T d = v[0][l];
+          if (!((dead >> l) & 1u) && d < -eps_d && better(d, j, bv, bk)) { bv = d; bk = j; bp = j; }
+        }
+      }
+    }
+    red.argmin(bv, bk, bp);
+    return bp;
+  }
+
+  __device__ __forceinline__ int ratio(int s, bool bland, T eps_p, T& theta) {
+    publish_col(s);
+    if (gc == 0) {
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) rhsbuf[row_of(k)] = v[k][0];
+    }
+    __syncthreads();
+    col_ready = s;
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    for (int i = tid; i < R; i += NT) {
+      if (i < 2) continue;
+      const T a = colbuf[i];
+      if (a > eps_p) {
+        const T beta = rhsbuf[i];
+        const T th = (beta > T(0) ? beta : T(0)) / a;
+        const int key = bland ? bs[i] : i;
+        if (better(th, key, bv, bk)) { bv = th; bk = key; bp = i; }
+      }
+    }
+    red.argmin(bv, bk, bp);
+    theta = bv;
+    return bp;
+  }
+
+  __device__ __forceinline__ void pivot(int r, int s) {
+    if (col_ready != s) {
+      __syncthreads();
+      publish_col(s);
+      __syncthreads();
+    }
+    const T p = uniform(colbuf[r]);
+    const T rp = T(1) / p;
+    const int kr = r / TR, grr = r - kr * TR;
+    const int ls = s / TC, gcs = s - ls * TC;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      if (k == kr) {
+        if (gr == grr) {
+#pragma unroll
+          for (int l = 0; l < CPT; ++l) {
+            const int j = col_of(l);
+            rowbuf[j] = (j == s) ? rp : v[k][l] * rp;
+          }
+        }
+      }
+    }
+    if (tid == 0) { const int t = bs[r]; bs[r] = nb[s]; nb[s] = t; }
+    __syncthreads();
+    T col[RPT], prow[CPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) col[k] = -colbuf[row_of(k)];
+#pragma unroll
+    for (int l = 0; l < CPT; ++l) prow[l] = rowbuf[col_of(l)];
+#pragma unroll
+    for (int l = 0; l < CPT; ++l) {
+      if (l == ls) {
+        const bool own = gc == gcs;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) v[k][l] = own ? T(0) : v[k][l];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+#pragma unroll
+      for (int l = 0; l < CPT; ++l) v[k][l] = fma_t(col[k], prow[l], v[k][l]);
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      if (k == kr) {
+        const bool own = gr == grr;
+#pragma unroll
+        for (int l = 0; l < CPT; ++l) v[k][l] = own ? prow[l] : v[k][l];
+      }
+    }
+    col_ready = -1;
+  }
+
+  __device__ __forceinline__ int absmax_col(int r, T eps_p) {
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    const int kr = r / TR, grr = r - kr * TR;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      if (k == kr && gr == grr) {
+#pragma unroll
+        for (int l = 0; l < CPT; ++l) {
+          const int j = col_of(l);
+          const T key = -tabs(v[k][l]);
+          if (!((dead >> l) & 1u) && better(key, j, bv, bk)) { bv = key; bk = j; bp = j; }
+        }
+      }
+    }
+    red.argmin(bv, bk, bp);
+    if (bp < 0 || !(-bv > eps_p)) return -1;
+    return bp;
+  }
+
+  __device__ __forceinline__ void kill_col(int j) {
+    const int lj = j / TC, gcj = j - lj * TC;
+#pragma unroll
+    for (int l = 0; l < CPT; ++l) {
+      if (l == lj) {
+        const bool own = gc == gcj;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) v[k][l] = own ? T(0) : v[k][l];
+        if (own) dead |= 1u << l;
+      }
+    }
+  }
+
+  __device__ __forceinline__ T get(int i, int j) {
+    const int ki = i / TR, gri = i - ki * TR, lj = j / TC, gcj = j - lj * TC;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+#pragma unroll
+      for (int l = 0; l < CPT; ++l)
+        if (k == ki && l == lj && gr == gri && gc == gcj) *scal = v[k][l];
+    __syncthreads();
+    const T r = uniform(*scal);
+    __syncthreads();
+    return r;
+  }
+
+  __device__ __forceinline__ void set(int i, int j, T val) {
+    const int ki = i / TR, gri = i - ki * TR, lj = j / TC, gcj = j - lj * TC;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+#pragma unroll
+      for (int l = 0; l < CPT; ++l)
+        if (k == ki && l == lj) v[k][l] = (gr == gri && gc == gcj) ? val : v[k][l];
+  }
+
+  __device__ __forceinline__ void mark_redund(int r) {
+    __syncthreads();
+    if (tid == 0) redund[r] = 1;
+    set(r, 0, T(0));
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ int min_rhs_row() {
+    __syncthreads();
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    if (gc == 0) {
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int i = row_of(k);
+        if (i >= 2 && i < R && !redund[i]) {
+          const T val = v[k][0];
+          if (better(val, i, bv, bk)) { bv = val; bk = i; bp = i; }
+        }
+      }
+    }
+    red.argmin(bv, bk, bp);
+    return bp;
+  }
+
+  __device__ __forceinline__ void setup_x0() {
+    __syncthreads();
+    const int x0 = C - 1, lx = x0 / TC, gcx = x0 - lx * TC;
+#pragma unroll
+    for (int l = 0; l < CPT; ++l) {
+      if (l == lx && gc == gcx) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+          const int i = row_of(k);
+          v[k][l] = (i >= 2 && i < R) ? (redund[i] ? T(0) : T(-1)) : (i == 1 ? T(1) : T(0));
+        }
+        dead &= ~(1u << l);
+      }
+    }
+    if (gr == 1) {
+#pragma unroll
+      for (int l = 0; l < CPT; ++l)
+        if (col_of(l) != x0) v[0][l] = T(0);
+    }
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ int find_basic(int var) {
+    __syncthreads();
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    for (int i = 2 + tid; i < R; i += NT)
+      if (bs[i] == var) { bv = T(0); bk = i; bp = i; }
+    red.argmin(bv, bk, bp);
+    return bp;
+  }
+
+  __device__ __forceinline__ int find_nonbasic(int var) {
+    __syncthreads();
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    for (int j = 1 + tid; j < C; j += NT)
+      if (nb[j] == var) { bv = T(0); bk = j; bp = j; }
+    red.argmin(bv, bk, bp);
+    return bp;
+  }
+
+  __device__ __forceinline__ int nb_of(int j) {
+    __syncthreads();
+    return uniform(nb[j]);
+  }
+
+  __device__ __forceinline__ void write_outputs(int lp, const LpDesc& d, const Out& out, int status, int it,
+                                int p1, int unb, T wfin) {
+    __syncthreads();
+    if (tid == 0) {
+      out.status[lp] = status;
+      out.pivots[lp] = it;
+      out.p1[lp] = p1;
+      out.unb[lp] = unb;
+      out.obj[lp] = (double)v[0][0];
+      out.wfin[lp] = (double)wfin;
+    }
+    double* x = out.x + d.x;
+    int32_t* basis = out.basis + d.basis;
+    for (int j = tid; j < n; j += NT) x[j] = 0.0;
+    for (int i = tid; i < m; i += NT) basis[i] = bs[2 + i];
+    __syncthreads();
+    if (gc == 0) {
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int i = row_of(k);
+        if (i >= 2 && i < R) {
+          const int var = bs[i];
+          if (var >= 0 && var < n) x[var] = (double)v[k][0];
+        }
+      }
+    }
+  }
+};
+
+template <typename T, int NW, int TC, int RPT, int CPT>
+__global__ __launch_bounds__(NW * 64) void k_tile(const LpDesc* __restrict__ desc,
+                                                  const double* __restrict__ pool,
+                                                  const int8_t* __restrict__ spool,
+                                                  const int32_t* __restrict__ idx, int32_t count,
+                                                  const int32_t* count_dev, int32_t* queue,
+                                                  Out out, Opt<T> o,
+                                                  const int32_t* __restrict__ warm_pool) {
+  using Tab = TileTab<T, NW, TC, RPT, CPT>;
+  __shared__ T colbuf[Tab::RCAP], rhsbuf[Tab::RCAP], rowbuf[Tab::CCAP], scal[2];
+  __shared__ int bs[Tab::RCAP], nb[Tab::CCAP];
+  __shared__ uint8_t redund[Tab::RCAP], target[Tab::RCAP + Tab::CCAP];
+  __shared__ RedSlot<T> slots[2 * NW];
+  __shared__ int s_q;
+  Tab t;
+  t.tid = threadIdx.x;
+  t.gr = threadIdx.x / TC;
+  t.gc = threadIdx.x - t.gr * TC;
+  t.colbuf = colbuf; t.rhsbuf = rhsbuf; t.rowbuf = rowbuf; t.scal = scal;
+  t.bs = bs; t.nb = nb; t.redund = redund; t.target = target;
+  t.red.init(slots);
+  if (count_dev) count = *count_dev;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
+    __syncthreads();
+    const int q = uniform(s_q);
+    if (q >= count) break;
+    const int lp = idx ? idx[q] : q;
+    const LpDesc d = desc[lp];
+    t.m = d.m; t.n = d.n; t.R = d.m + 2; t.C = d.n + 2;
+    const int8_t* sense = d.s >= 0 ? spool + d.s : nullptr;
+    t.init(pool + d.a, pool + d.b, pool + d.c, sense);
+    const int32_t* warm = warm_pool ? warm_pool + d.basis : nullptr;
+    t.set_target(warm);
+    int status, it, p1, unb;
+    T wfin;
+    run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, warm);
+    t.write_outputs(lp, d, out, status, it, p1, unb, wfin);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// variant table: (waves per LP, TC, RPT, CPT); capacity (64*NW/TC*RPT) x (TC*CPT)
+#define BLPG_TILE_VARIANTS(X) \
+  X(0, 1, 8, 4, 5)            \
+  X(1, 1, 8, 7, 7)            \
+  X(2, 2, 16, 8, 7)           \
+  X(3, 2, 16, 10, 6)          \
+  X(4, 4, 16, 7, 7)           \
+  X(5, 8, 16, 6, 9)           \
+  X(6, 8, 16, 7, 13)
+
+constexpr int kTileVariants = 7;
+
+int tile_variant_count(int) { return kTileVariants; }
+
+TileVariant tile_variant(int, int v) {
+  switch (v) {
+#define X(id, nw, tc, rpt, cpt) \
+  case id: return TileVariant{nw, (nw * 64) / tc, rpt, cpt};
+    BLPG_TILE_VARIANTS(X)
+#undef X
+    default: return TileVariant{1, 8, 1, 1};
+  }
+}
+
+template <typename T>
+static const void* tile_fn(int v) {
+  switch (v) {
+#define X(id, nw, tc, rpt, cpt) \
+  case id: return reinterpret_cast<const void*>(&k_tile<T, nw, tc, rpt, cpt>);
+    BLPG_TILE_VARIANTS(X)
+#undef X
+    default: return nullptr;
+  }
+}
+
+int tile_max_blocks_per_cu(int precision, int v) {
+  const void* fn = precision == 0 ? tile_fn<double>(v) : tile_fn<float>(v);
+  if (!fn) return 0;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, tile_variant(precision, v).nw * 64,
+                                                   0) != hipSuccess)
+    return 0;
+  return nb;
+}
+
+hipError_t launch_tile(const TileLaunch& L, hipStream_t st) {
+  const void* fn = L.precision == 0 ? tile_fn<double>(L.variant) : tile_fn<float>(L.variant);
+  if (!fn) return hipErrorInvalidValue;
+  const int nt = tile_variant(L.precision, L.variant).nw * 64;
+  if (L.precision == 0) {
+    Opt<double> o{L.eps_d, L.eps_p, L.tol, L.max_iter, L.bland_after};
+    void* args[] = {(void*)&L.desc, (void*)&L.pool, (void*)&L.spool, (void*)&L.idx,
+                    (void*)&L.count, (void*)&L.count_dev, (void*)&L.queue, (void*)&L.out,
+                    (void*)&o, (void*)&L.warm};
+    return hipLaunchKernel(fn, dim3(L.grid), dim3(nt), args, 0, st);
+  }
+  Opt<float> o{(float)L.eps_d, (float)L.eps_p, (float)L.tol, L.max_iter, L.bland_after};
+  void* args[] = {(void*)&L.desc, (void*)&L.pool, (void*)&L.spool, (void*)&L.idx,
+                  (void*)&L.count, (void*)&L.count_dev, (void*)&L.queue, (void*)&L.out,
+                  (void*)&o, (void*)&L.warm};
+  return hipLaunchKernel(fn, dim3(L.grid), dim3(nt), args, 0, st);
+}
 
 }  // namespace blpg

@@ -1,17 +1,24 @@
 // kernel_verify.hip -- FP64 check of final bases (no reference counterpart: the reference
 // returns only T[M-2][N-1], simplex.cu:380, and never checks it).
 //
-// One workgroup per LP, persistent over a work queue.  For an LP solved to OPTIMAL:
-//   B x_B = b'  and  B^T y = c_B  by LU with partial pivoting (B in LDS when it fits,
-//   else in an HBM slab), then
-//   primal residual = max(-x_j, -s_i ('<='), |s_i| ('='), |b' - A'x - s|) / (1 + |b|_inf)
-//   dual residual   = max over nonbasic, non-fixed variables of (c_j - y^T a'_j)_+ / (1 + |c|_inf)
-// and the FP64 objective c^T x.  For UNBOUNDED: the ray of the entering variable v,
-// d = B^-1 a'_v, must keep every basic variable feasible (d <= tol, |d| <= tol for fixed
-// slacks) and improve the objective (c_v - y^T a'_v > tol).
-// An FP32 result that fails (or an FP32 INFEASIBLE whose phase-I objective is not
-// clearly negative, or an iteration-limit / numeric status) is appended to a fallback
-// list that the FP64 kernel re-solves.
+// One wavefront per LP, persistent over a work queue, FP64 throughout.  A basis of m
+// variable ids splits into basic structurals S and basic slacks; the rows P whose slack is
+// nonbasic satisfy |P| = |S| = k.  With A', b' the LP with '>=' rows negated and
+// M = A'[P, S] (k x k), the basic solution and duals are
+//   x_S = M^-1 b'_P,   s_Q = b'_Q - A'[Q, S] x_S   (Q: rows with a basic slack),
+//   y_P = M^-T c_S,    y_Q = 0,
+// so only a k x k LU (partial pivoting, lowest row on ties) is needed, not an m x m one;
+// M sits in LDS when k <= 64 (the common case: k averages 20-60 on the mixed-size batch),
+// else in an HBM slab owned by the wavefront.
+// Checks, relative to (1 + |b|_inf + |A|_max |x|_inf) and (1 + |c|_inf + |A|_max |y|_inf):
+//   primal  x_S >= 0, s_Q >= 0 ('<=') / s_Q = 0 ('='), |b'_P - A'[P,S] x_S|
+//   dual    c_j - y_P^T A'[P, j] <= 0 for nonbasic structurals, -y_i <= 0 for nonbasic
+//           slacks of '<=' rows (fixed '=' slacks are free)
+//   OPTIMAL objective c_S^T x_S;  UNBOUNDED: the entering variable v's ray
+//           d = B^-1 a'_v must satisfy d <= tol (|d| <= tol on basic '=' slacks) and
+//           c_v - y^T a'_v > tol.
+// FP32 results that fail (or FP32 INFEASIBLE whose phase-I objective is not clearly
+// negative, or iteration-limit / numeric statuses) are appended to the fallback list.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -23,84 +30,150 @@ namespace blpg {
 
 namespace {
 
-constexpr int VNT = 256;
-constexpr int VNW = VNT / 64;
+constexpr int KLDS = 64;  // bases with k <= 64 basic structurals factor M in LDS
 
 struct VerifyLayout {
-  size_t B, piv, xb, y, d, inb, red, total;
-  __host__ __device__ VerifyLayout(bool lds, int mcap, int ncap) {
+  size_t M, S, P, pos, rhs, y, d, inb, total;
+  __host__ __device__ VerifyLayout(bool, int mcap, int ncap) {
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
     size_t o = 0;
-    B = o; if (lds) o += al(sizeof(double) * (size_t)mcap * mcap);
-    piv = o; o += al(sizeof(int) * mcap);
-    xb = o; o += al(sizeof(double) * mcap);
-    y = o; o += al(sizeof(double) * mcap);
-    d = o; o += al(sizeof(double) * mcap);
-    inb = o; o += al((size_t)mcap + ncap + 1);
-    red = o; o += al(sizeof(RedSlot<double>) * 2 * VNW + sizeof(double) * 2 * VNW);
+    M = o; o += al(sizeof(double) * (size_t)KLDS * KLDS);
+    S = o; o += al(sizeof(int) * mcap);      // basic structural ids (column order of M)
+    P = o; o += al(sizeof(int) * mcap);      // rows with nonbasic slack (row order of M), then Q
+    pos = o; o += al(sizeof(int) * mcap);    // LU row permutation
+    rhs = o; o += al(sizeof(double) * mcap); // x_S
+    y = o; o += al(sizeof(double) * mcap);   // y_P
+    d = o; o += al(sizeof(double) * mcap);   // ray
+    inb = o; o += al((size_t)mcap + ncap + 16);
     total = o;
   }
 };
 
-__device__ double block_sum(double v, double* slots, int& phase) {
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  double* s = slots + phase * VNW;
-  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = s[0];
-#pragma unroll
-  for (int q = 1; q < VNW; ++q) r += s[q];
-  phase ^= 1;
-  return r;
+  return uniform(v);
 }
-
-__device__ double block_max(double v, double* slots, int& phase) {
+__device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-  double* s = slots + phase * VNW;
-  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = s[0];
+  return uniform(v);
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
 #pragma unroll
-  for (int q = 1; q < VNW; ++q) r = fmax(r, s[q]);
-  phase ^= 1;
-  return r;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return uniform(v);
+}
+
+// M (k x k, row-major, stride k) := P L U in place; perm[c] = row swapped into c.
+// Returns false when a pivot is below 1e-13 * max|M|.
+__device__ bool lu64(double* Mx, int k, int* perm, double amax) {
+  const int lane = threadIdx.x;
+  for (int c = 0; c < k; ++c) {
+    double bv = DBL_MAX;
+    int bk = INT_MAX, bp = -1;
+    for (int i = c + lane; i < k; i += 64) {
+      const double v = -fabs(Mx[i * k + c]);
+      if (better(v, i, bv, bk)) { bv = v; bk = i; bp = i; }
+    }
+    wave_argmin(bv, bk, bp);
+    if (!(-bv > 1e-13 * (amax > 0 ? amax : 1.0))) return false;
+    if (lane == 0) perm[c] = bp;
+    if (bp != c)
+      for (int j = lane; j < k; j += 64) {
+        const double t = Mx[c * k + j];
+        Mx[c * k + j] = Mx[bp * k + j];
+        Mx[bp * k + j] = t;
+      }
+    __syncthreads();
+    const double dc = uniform(Mx[c * k + c]);
+    for (int i = c + 1 + lane; i < k; i += 64) Mx[i * k + c] /= dc;
+    __syncthreads();
+    // trailing update, lanes tiled 8 x 8 over (row, column)
+    const int li = lane >> 3, lj = lane & 7;
+    for (int i = c + 1 + li; i < k; i += 8) {
+      const double l = Mx[i * k + c];
+      if (l == 0.0) continue;
+      for (int j = c + 1 + lj; j < k; j += 8) Mx[i * k + j] = fma(-l, Mx[c * k + j], Mx[i * k + j]);
+    }
+    __syncthreads();
+  }
+  return true;
+}
+
+// z := M^-1 z  (z in LDS, k entries, row order of M)
+__device__ void lu_solve64(const double* Mx, int k, const int* perm, double* z) {
+  const int lane = threadIdx.x;
+  if (lane == 0)
+    for (int c = 0; c < k; ++c)
+      if (perm[c] != c) { const double t = z[c]; z[c] = z[perm[c]]; z[perm[c]] = t; }
+  __syncthreads();
+  for (int c = 0; c < k; ++c) {
+    const double zc = uniform(z[c]);
+    for (int i = c + 1 + lane; i < k; i += 64) z[i] = fma(-Mx[i * k + c], zc, z[i]);
+    __syncthreads();
+  }
+  for (int c = k - 1; c >= 0; --c) {
+    const double zc = uniform(z[c]) / uniform(Mx[c * k + c]);
+    __syncthreads();
+    if (lane == 0) z[c] = zc;
+    for (int i = lane; i < c; i += 64) z[i] = fma(-Mx[i * k + c], zc, z[i]);
+    __syncthreads();
+  }
+}
+
+// z := M^-T z
+__device__ void lu_solve_t64(const double* Mx, int k, const int* perm, double* z) {
+  const int lane = threadIdx.x;
+  for (int c = 0; c < k; ++c) {  // U^T w = z
+    const double zc = uniform(z[c]) / uniform(Mx[c * k + c]);
+    __syncthreads();
+    if (lane == 0) z[c] = zc;
+    for (int i = c + 1 + lane; i < k; i += 64) z[i] = fma(-Mx[c * k + i], zc, z[i]);
+    __syncthreads();
+  }
+  for (int c = k - 1; c >= 0; --c) {  // L^T u = w
+    const double zc = uniform(z[c]);
+    for (int i = lane; i < c; i += 64) z[i] = fma(-Mx[c * k + i], zc, z[i]);
+    __syncthreads();
+  }
+  if (lane == 0)
+    for (int c = k - 1; c >= 0; --c)
+      if (perm[c] != c) { const double t = z[c]; z[c] = z[perm[c]]; z[perm[c]] = t; }
+  __syncthreads();
 }
 
 }  // namespace
 
-template <bool LDS>
-__global__ __launch_bounds__(VNT) void k_verify(const LpDesc* __restrict__ desc,
-                                                const double* __restrict__ pool,
-                                                const int8_t* __restrict__ spool,
-                                                const int32_t* __restrict__ idx, int32_t count,
-                                                const int32_t* count_dev, Out out, Check chk,
-                                                double tol, int fp32_source, double* ws,
-                                                int64_t ws_stride, int mcap, int ncap,
-                                                int32_t* queue, int32_t* fail_list,
-                                                int32_t* fail_count) {
+__global__ __launch_bounds__(64) void k_verify(const LpDesc* __restrict__ desc,
+                                               const double* __restrict__ pool,
+                                               const int8_t* __restrict__ spool,
+                                               const int32_t* __restrict__ idx, int32_t count,
+                                               const int32_t* count_dev, Out out, Check chk,
+                                               double tol, int fp32_source, double* ws,
+                                               int64_t ws_stride, int mcap, int ncap,
+                                               int32_t* queue, int32_t* fail_list,
+                                               int32_t* fail_count) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int s_q, s_bad;
-  const VerifyLayout L(LDS, mcap, ncap);
-  double* Bm = LDS ? (double*)(smem + L.B) : ws + (int64_t)blockIdx.x * ws_stride;
-  int* piv = (int*)(smem + L.piv);
-  double* xb = (double*)(smem + L.xb);
-  double* y = (double*)(smem + L.y);
+  __shared__ int s_q, s_k;
+  const VerifyLayout L(true, mcap, ncap);
+  double* Mlds = (double*)(smem + L.M);
+  double* Mws = ws + (int64_t)blockIdx.x * ws_stride;
+  int* Sl = (int*)(smem + L.S);
+  int* Pl = (int*)(smem + L.P);
+  int* perm = (int*)(smem + L.pos);
+  double* xs = (double*)(smem + L.rhs);
+  double* yp = (double*)(smem + L.y);
   double* dv = (double*)(smem + L.d);
   uint8_t* inb = smem + L.inb;
-  Reducer<double, VNW> red;
-  red.init((RedSlot<double>*)(smem + L.red));
-  double* fslots = (double*)(smem + L.red + sizeof(RedSlot<double>) * 2 * VNW);
-  int fphase = 0;
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
   if (count_dev) count = *count_dev;
 
   for (;;) {
     __syncthreads();
-    if (tid == 0) s_q = atomicAdd(queue, 1);
+    if (lane == 0) s_q = atomicAdd(queue, 1);
     __syncthreads();
-    const int q = s_q;
+    const int q = uniform(s_q);
     if (q >= count) break;
     const int lp = idx ? idx[q] : q;
     const LpDesc D = desc[lp];
@@ -111,229 +184,158 @@ __global__ __launch_bounds__(VNT) void k_verify(const LpDesc* __restrict__ desc,
     const int8_t* sense = D.s >= 0 ? spool + D.s : nullptr;
     const int status = out.status[lp];
     const int32_t* basis = out.basis + D.basis;
+    auto sg = [&](int i) { return (sense && sense[i] == 2) ? -1.0 : 1.0; };
+    auto is_eq = [&](int i) { return sense && sense[i] == 1; };
 
-    double bmax = 0, cmax = 0;
-    for (int i = tid; i < m; i += VNT) bmax = fmax(bmax, fabs(b[i]));
-    for (int j = tid; j < n; j += VNT) cmax = fmax(cmax, fabs(c[j]));
-    bmax = block_max(bmax, fslots, fphase);
-    cmax = block_max(cmax, fslots, fphase);
+    double bmax = 0, cmax = 0, am = 0;
+    for (int i = lane; i < m; i += 64) bmax = fmax(bmax, fabs(b[i]));
+    for (int j = lane; j < n; j += 64) cmax = fmax(cmax, fabs(c[j]));
+    for (int e = lane; e < m * n; e += 64) am = fmax(am, fabs(A[e]));
+    bmax = wave_max(bmax);
+    cmax = wave_max(cmax);
+    am = wave_max(am);
 
-    int ok = 2;  // not checkable
+    int ok = 2;
     double rp = -1, rd = -1, zobj = out.obj[lp];
     const bool checkable = status == kStatusOptimal || status == kStatusUnbounded;
     if (!checkable) {
       if (fp32_source)
         ok = (status == kStatusInfeasible && out.wfin[lp] < -1e-3 * (1.0 + bmax)) ? 1 : 0;
     } else {
-      // ---- basis matrix B (row-major m x m), columns of [A' | I] ----
-      for (int v = tid; v < n + m; v += VNT) inb[v] = 0;
-      if (tid == 0) s_bad = 0;
+      // ---- split the basis: S (structurals), P (rows with nonbasic slack), Q ----
+      for (int v = lane; v < n + m; v += 64) inb[v] = 0;
       __syncthreads();
-      for (int k = tid; k < m; k += VNT) {
-        const int v = basis[k];
-        if (v < 0 || v >= n + m) s_bad = 1;
+      int bad = 0;
+      for (int kk = lane; kk < m; kk += 64) {
+        const int v = basis[kk];
+        if (v < 0 || v >= n + m) bad = 1;
         else inb[v] = 1;
       }
+      bad = wave_sum_i(bad);
       __syncthreads();
-      int bad = s_bad;
-      if (!bad) {
-        for (int e = tid; e < m * m; e += VNT) {
-          const int i = e / m, k = e - (e / m) * m;
-          const int v = basis[k];
-          const double sg = (sense && sense[i] == 2) ? -1.0 : 1.0;
-          Bm[e] = v < n ? sg * A[(int64_t)i * n + v] : (v - n == i ? 1.0 : 0.0);
-        }
-        __syncthreads();
-        double amax = 0;
-        for (int e = tid; e < m * m; e += VNT) amax = fmax(amax, fabs(Bm[e]));
-        amax = block_max(amax, fslots, fphase);
-        // ---- LU with partial pivoting (lowest row index on ties) ----
-        for (int k = 0; k < m && !bad; ++k) {
-          double bv = DBL_MAX;
-          int bk = INT_MAX, bp = -1;
-          for (int i = k + tid; i < m; i += VNT) {
-            const double v = -fabs(Bm[i * m + k]);
-            if (better(v, i, bv, bk)) { bv = v; bk = i; bp = i; }
-          }
-          red.argmin(bv, bk, bp);
-          if (!(-bv > 1e-13 * (amax > 0 ? amax : 1.0))) { bad = 1; break; }
-          if (tid == 0) piv[k] = bp;
-          if (bp != k)
-            for (int j = tid; j < m; j += VNT) {
-              const double t = Bm[k * m + j];
-              Bm[k * m + j] = Bm[bp * m + j];
-              Bm[bp * m + j] = t;
-            }
-          __syncthreads();
-          const double dk = Bm[k * m + k];
-          for (int i = k + 1 + tid; i < m; i += VNT) Bm[i * m + k] /= dk;
-          __syncthreads();
-          const int rem = m - k - 1;
-          for (int e = tid; e < rem * rem; e += VNT) {
-            const int i = k + 1 + e / rem, j = k + 1 + (e - (e / rem) * rem);
-            Bm[i * m + j] = fma(-Bm[i * m + k], Bm[k * m + j], Bm[i * m + j]);
-          }
-          __syncthreads();
-        }
+      if (lane == 0) {
+        int ns = 0, np = 0;
+        for (int j = 0; j < n; ++j) if (inb[j]) Sl[ns++] = j;
+        for (int i = 0; i < m; ++i) if (!inb[n + i]) Pl[np++] = i;
+        int nq = np;
+        for (int i = 0; i < m; ++i) if (inb[n + i]) Pl[nq++] = i;
+        s_k = (ns == np) ? ns : -1;
       }
-      if (bad) {
+      __syncthreads();
+      const int k = uniform(s_k);
+      if (bad || k < 0) {
         ok = 0;
       } else {
-        // ---- B x_B = b' ----
-        for (int i = tid; i < m; i += VNT) xb[i] = (sense && sense[i] == 2) ? -b[i] : b[i];
-        __syncthreads();
-        if (tid == 0)
-          for (int k = 0; k < m; ++k)
-            if (piv[k] != k) { const double t = xb[k]; xb[k] = xb[piv[k]]; xb[piv[k]] = t; }
-        __syncthreads();
-        for (int j = 0; j < m; ++j) {
-          const double xj = xb[j];
-          for (int i = j + 1 + tid; i < m; i += VNT) xb[i] = fma(-Bm[i * m + j], xj, xb[i]);
-          __syncthreads();
+        auto check = [&](double* Mx) {
+        // ---- M = A'[P, S] and its LU ----
+        double mmax = 0;
+        for (int e = lane; e < k * k; e += 64) {
+          const int a = e / k, bcol = e - (e / k) * k;
+          const int i = Pl[a];
+          const double v = sg(i) * A[(int64_t)i * n + Sl[bcol]];
+          Mx[e] = v;
+          mmax = fmax(mmax, fabs(v));
         }
-        for (int j = m - 1; j >= 0; --j) {
-          const double xj = xb[j] / Bm[j * m + j];
-          __syncthreads();
-          if (tid == 0) xb[j] = xj;
-          for (int i = tid; i < j; i += VNT) xb[i] = fma(-Bm[i * m + j], xj, xb[i]);
-          __syncthreads();
-        }
-        // ---- B^T y = c_B ----
-        for (int k = tid; k < m; k += VNT) y[k] = basis[k] < n ? c[basis[k]] : 0.0;
+        mmax = wave_max(mmax);
         __syncthreads();
-        for (int j = 0; j < m; ++j) {  // U^T z = c_B
-          const double zj = y[j] / Bm[j * m + j];
-          __syncthreads();
-          if (tid == 0) y[j] = zj;
-          for (int i = j + 1 + tid; i < m; i += VNT) y[i] = fma(-Bm[j * m + i], zj, y[i]);
-          __syncthreads();
-        }
-        for (int j = m - 1; j >= 0; --j) {  // L^T w = z
-          const double wj = y[j];
-          for (int i = tid; i < j; i += VNT) y[i] = fma(-Bm[j * m + i], wj, y[i]);
-          __syncthreads();
-        }
-        if (tid == 0)
-          for (int k = m - 1; k >= 0; --k)
-            if (piv[k] != k) { const double t = y[k]; y[k] = y[piv[k]]; y[piv[k]] = t; }
-        __syncthreads();
-
-        // ---- x (structural) and slacks ----
-        double* x = chk.x + D.x;
-        for (int j = tid; j < n; j += VNT) x[j] = 0.0;
-        __syncthreads();
-        for (int k = tid; k < m; k += VNT)
-          if (basis[k] < n) x[basis[k]] = xb[k];
-        __syncthreads();
-        // primal: bounds of basic variables, then row residuals (one wave per row)
-        double p = 0;
-        for (int k = tid; k < m; k += VNT) {
-          const int v = basis[k];
-          const bool fixed = v >= n && sense && sense[v - n] == 1;
-          p = fmax(p, fixed ? fabs(xb[k]) : -xb[k]);
-        }
-        // slack values (dv is free until the unbounded check)
-        for (int i = tid; i < m; i += VNT) dv[i] = 0.0;
-        __syncthreads();
-        for (int k = tid; k < m; k += VNT)
-          if (basis[k] >= n) dv[basis[k] - n] = xb[k];
-        __syncthreads();
-        const int lane = tid & 63, w = tid >> 6;
-        for (int i = w; i < m; i += VNW) {
-          double acc = 0;
-          for (int j = lane; j < n; j += 64) acc = fma(A[(int64_t)i * n + j], x[j], acc);
-#pragma unroll
-          for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-          const double sg = (sense && sense[i] == 2) ? -1.0 : 1.0;
-          if (lane == 0) p = fmax(p, fabs(sg * b[i] - dv[i] - sg * acc));
-        }
-        // scale-aware normalisation: |r| / (1 + |b|_inf + |A|_max |x|_inf)
-        double xm = 0;
-        for (int k = tid; k < m; k += VNT) xm = fmax(xm, fabs(xb[k]));
-        xm = block_max(xm, fslots, fphase);
-        double ym = 0;
-        for (int k = tid; k < m; k += VNT) ym = fmax(ym, fabs(y[k]));
-        ym = block_max(ym, fslots, fphase);
-        double am = 0;
-        for (int e = tid; e < m * n; e += VNT) am = fmax(am, fabs(A[e]));
-        am = block_max(am, fslots, fphase);
-        rp = block_max(p, fslots, fphase) / (1.0 + bmax + am * xm);
-        // dual: reduced costs of nonbasic variables (one thread per column)
-        double dmax = 0;
-        for (int j = tid; j < n; j += VNT) {
-          double r = c[j];
-          for (int i = 0; i < m; ++i) {
-            const double sg = (sense && sense[i] == 2) ? -1.0 : 1.0;
-            r = fma(-y[i], sg * A[(int64_t)i * n + j], r);
-          }
-          if (!inb[j]) dmax = fmax(dmax, r);
-        }
-        for (int i = tid; i < m; i += VNT)
-          if (!inb[n + i] && !(sense && sense[i] == 1)) dmax = fmax(dmax, -y[i]);
-        rd = block_max(dmax, fslots, fphase) / (1.0 + cmax + am * ym);
-        double zp = 0;
-        for (int j = tid; j < n; j += VNT) zp = fma(c[j], x[j], zp);
-        const double zsum = block_sum(zp, fslots, fphase);
-
-        if (status == kStatusOptimal) {
-          zobj = zsum;
-          ok = (rp <= tol && rd <= tol) ? 1 : 0;
+        if (k > 0 && !lu64(Mx, k, perm, mmax)) {
+          ok = 0;
         } else {
-          // ---- unbounded: ray of the entering variable ----
-          const int v = out.unb[lp];
+          // x_S = M^-1 b'_P
+          for (int a = lane; a < k; a += 64) xs[a] = sg(Pl[a]) * b[Pl[a]];
           __syncthreads();
-          bool good = v >= 0 && v < n + m && !inb[v] && rp <= tol;
-          if (good) {
-            for (int i = tid; i < m; i += VNT) {
-              const double sg = (sense && sense[i] == 2) ? -1.0 : 1.0;
-              dv[i] = v < n ? sg * A[(int64_t)i * n + v] : (v - n == i ? 1.0 : 0.0);
-            }
-            __syncthreads();
-            if (tid == 0)
-              for (int k = 0; k < m; ++k)
-                if (piv[k] != k) { const double t = dv[k]; dv[k] = dv[piv[k]]; dv[piv[k]] = t; }
-            __syncthreads();
-            for (int j = 0; j < m; ++j) {
-              const double dj = dv[j];
-              for (int i = j + 1 + tid; i < m; i += VNT) dv[i] = fma(-Bm[i * m + j], dj, dv[i]);
-              __syncthreads();
-            }
-            for (int j = m - 1; j >= 0; --j) {
-              const double dj = dv[j] / Bm[j * m + j];
-              __syncthreads();
-              if (tid == 0) dv[j] = dj;
-              for (int i = tid; i < j; i += VNT) dv[i] = fma(-Bm[i * m + j], dj, dv[i]);
-              __syncthreads();
-            }
-            // reduced cost of v
-            double rv = 0;
-            if (v < n) {
-              double part = 0;
-              for (int i = tid; i < m; i += VNT) {
-                const double sg = (sense && sense[i] == 2) ? -1.0 : 1.0;
-                part = fma(y[i], sg * A[(int64_t)i * n + v], part);
-              }
-              rv = c[v] - block_sum(part, fslots, fphase);
-            } else {
-              rv = -y[v - n];
-            }
-            double viol = 0;
-            const double dtol = tol * (1.0 + bmax) * 1e3;
-            for (int k = tid; k < m; k += VNT) {
-              const int bv2 = basis[k];
-              const bool fixed = bv2 >= n && sense && sense[bv2 - n] == 1;
-              viol = fmax(viol, fixed ? fabs(dv[k]) : dv[k]);
-            }
-            viol = block_max(viol, fslots, fphase);
-            good = viol <= dtol && rv > tol * (1.0 + cmax);
+          if (k > 0) lu_solve64(Mx, k, perm, xs);
+          // y_P = M^-T c_S
+          for (int a = lane; a < k; a += 64) yp[a] = c[Sl[a]];
+          __syncthreads();
+          if (k > 0) lu_solve_t64(Mx, k, perm, yp);
+          double xm = 0, ym = 0;
+          for (int a = lane; a < k; a += 64) { xm = fmax(xm, fabs(xs[a])); ym = fmax(ym, fabs(yp[a])); }
+          // primal: bounds of x_S, basic slacks s_Q, residual rows P
+          double pv = 0;
+          for (int a = lane; a < k; a += 64) pv = fmax(pv, -xs[a]);
+          for (int qq = k + lane; qq < m; qq += 64) {
+            const int i = Pl[qq];
+            double s = sg(i) * b[i];
+            for (int a = 0; a < k; ++a) s = fma(-sg(i) * A[(int64_t)i * n + Sl[a]], xs[a], s);
+            xm = fmax(xm, fabs(s));
+            pv = fmax(pv, is_eq(i) ? fabs(s) : -s);
           }
-          ok = good ? 1 : 0;
-          rd = -1;  // no dual solution for an unbounded LP
+          for (int a = lane; a < k; a += 64) {
+            const int i = Pl[a];
+            double r = sg(i) * b[i];
+            for (int e2 = 0; e2 < k; ++e2) r = fma(-sg(i) * A[(int64_t)i * n + Sl[e2]], xs[e2], r);
+            pv = fmax(pv, fabs(r));
+          }
+          xm = wave_max(xm);
+          ym = wave_max(ym);
+          rp = wave_max(pv) / (1.0 + bmax + am * xm);
+          // dual: reduced costs of nonbasic structurals and '<=' slacks
+          double dvv = 0;
+          for (int j = lane; j < n; j += 64) {
+            if (inb[j]) continue;
+            double r = c[j];
+            for (int a = 0; a < k; ++a) r = fma(-yp[a], sg(Pl[a]) * A[(int64_t)Pl[a] * n + j], r);
+            dvv = fmax(dvv, r);
+          }
+          for (int a = lane; a < k; a += 64)
+            if (!is_eq(Pl[a])) dvv = fmax(dvv, -yp[a]);
+          rd = wave_max(dvv) / (1.0 + cmax + am * ym);
+          // x (structural, FP64) and objective
+          double* x = chk.x + D.x;
+          for (int j = lane; j < n; j += 64) x[j] = 0.0;
+          __syncthreads();
+          double zp = 0;
+          for (int a = lane; a < k; a += 64) {
+            x[Sl[a]] = xs[a];
+            zp = fma(c[Sl[a]], xs[a], zp);
+          }
+          const double zsum = wave_sum(zp);
+          if (status == kStatusOptimal) {
+            zobj = zsum;
+            ok = (rp <= tol && rd <= tol) ? 1 : 0;
+          } else {
+            // ray of the entering variable v: d = B^-1 a'_v
+            const int v = out.unb[lp];
+            bool good = v >= 0 && v < n + m && !inb[v] && rp <= tol;
+            if (good) {
+              for (int a = lane; a < k; a += 64) {
+                const int i = Pl[a];
+                dv[a] = v < n ? sg(i) * A[(int64_t)i * n + v] : (v - n == i ? 1.0 : 0.0);
+              }
+              __syncthreads();
+              if (k > 0) lu_solve64(Mx, k, perm, dv);
+              double viol = 0, dmax = 0;
+              for (int a = lane; a < k; a += 64) { viol = fmax(viol, dv[a]); dmax = fmax(dmax, fabs(dv[a])); }
+              for (int qq = k + lane; qq < m; qq += 64) {
+                const int i = Pl[qq];
+                double s = v < n ? sg(i) * A[(int64_t)i * n + v] : (v - n == i ? 1.0 : 0.0);
+                for (int a = 0; a < k; ++a) s = fma(-sg(i) * A[(int64_t)i * n + Sl[a]], dv[a], s);
+                viol = fmax(viol, is_eq(i) ? fabs(s) : s);
+                dmax = fmax(dmax, fabs(s));
+              }
+              viol = wave_max(viol);
+              dmax = wave_max(dmax);
+              double part = 0;
+              if (v < n) {
+                for (int a = lane; a < k; a += 64) part = fma(yp[a], sg(Pl[a]) * A[(int64_t)Pl[a] * n + v], part);
+              } else {
+                for (int a = lane; a < k; a += 64) if (Pl[a] == v - n) part = yp[a];
+              }
+              const double rv = (v < n ? c[v] : 0.0) - wave_sum(part);
+              good = viol <= tol * (1.0 + dmax) && rv > tol * (1.0 + cmax + am * ym);
+            }
+            ok = good ? 1 : 0;
+            rd = -1;
+          }
         }
+        };
+        if (k <= KLDS) check(Mlds);
+        else check(Mws);
       }
     }
     __syncthreads();
-    if (tid == 0) {
+    if (lane == 0) {
       chk.ok[lp] = ok;
       chk.obj[lp] = zobj;
       chk.kkt_p[lp] = rp;
@@ -351,25 +353,22 @@ size_t verify_lds_bytes(int mcap, int ncap, bool lds) { return VerifyLayout(lds,
 
 int verify_max_blocks_per_cu(int mcap, int ncap, bool lds) {
   int nb = 0;
-  const void* fn = lds ? reinterpret_cast<const void*>(&k_verify<true>)
-                       : reinterpret_cast<const void*>(&k_verify<false>);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, VNT, verify_lds_bytes(mcap, ncap, lds)) !=
+  const void* fn = reinterpret_cast<const void*>(&k_verify);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, verify_lds_bytes(mcap, ncap, lds)) !=
       hipSuccess)
     return 0;
   return nb;
 }
 
 hipError_t launch_verify(const VerifyLaunch& V, hipStream_t st) {
-  const bool lds = V.ws == nullptr;
-  const size_t bytes = verify_lds_bytes(V.mcap, V.ncap, lds);
-  const void* fn = lds ? reinterpret_cast<const void*>(&k_verify<true>)
-                       : reinterpret_cast<const void*>(&k_verify<false>);
+  const size_t bytes = verify_lds_bytes(V.mcap, V.ncap, true);
+  const void* fn = reinterpret_cast<const void*>(&k_verify);
   void* args[] = {(void*)&V.desc, (void*)&V.pool, (void*)&V.spool, (void*)&V.idx,
                   (void*)&V.count, (void*)&V.count_dev, (void*)&V.out, (void*)&V.chk,
                   (void*)&V.tol, (void*)&V.fp32_source, (void*)&V.ws, (void*)&V.ws_stride,
                   (void*)&V.mcap, (void*)&V.ncap, (void*)&V.queue, (void*)&V.fail_list,
                   (void*)&V.fail_count};
-  return hipLaunchKernel(fn, dim3(V.grid), dim3(VNT), args, bytes, st);
+  return hipLaunchKernel(fn, dim3(V.grid), dim3(64), args, bytes, st);
 }
 
 }  // namespace blpg

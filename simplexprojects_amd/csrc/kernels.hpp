@@ -28,6 +28,7 @@ struct StreamLaunch {
   int64_t ws_stride;
   int rcap, ccap;      // largest m+2, n+2 of the bin
   int grid;
+  const int32_t* warm = nullptr;  // warm-start bases (basis pool layout), or nullptr
 };
 size_t stream_lds_bytes(int precision, bool lds_tab, int nt, int rcap, int ccap);
 int stream_max_blocks_per_cu(int precision, bool lds_tab, int nt, int rcap, int ccap);
@@ -48,6 +49,7 @@ struct TileLaunch {
   double eps_d, eps_p, tol;
   int32_t max_iter, bland_after;
   int grid;
+  const int32_t* warm = nullptr;  // warm-start bases (basis pool layout), or nullptr
 };
 struct TileVariant {
   int nw, tr, rpt, cpt;  // waves per LP, row groups, rows / columns per thread

@@ -38,18 +38,87 @@ __device__ __forceinline__ float inf_of<float>() {
   return __builtin_huge_valf();
 }
 
-// Butterfly argmin over the 64 lanes of a wavefront; every lane ends with the result.
+// Wave-uniform copies (SGPR) of values every lane already holds: lets the compiler turn
+// comparisons against pivot indices into scalar branches.
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uniform(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uniform(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// DPP lane movement inside 16-lane rows (gfx9 encodings): quad_perm [1,0,3,2] and
+// [2,3,0,1], row_half_mirror (lane i -> 7-i in each 8-lane half), row_mirror (i -> 15-i).
+// Once the lanes of a quad / half-row hold equal keys, the mirrors act as xor-4 / xor-8.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+  return __int_as_float(dpp<CTRL>(__float_as_int(x)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp<CTRL>((int)(b & 0xffffffffll));
+  const int hi = dpp<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int CTRL, typename T>
+__device__ __forceinline__ void dpp_step(T& v, int& k, int& p) {
+  const T ov = dpp<CTRL>(v);
+  const int ok = dpp<CTRL>(k);
+  const int op = dpp<CTRL>(p);
+  const bool b = better(ov, ok, v, k);
+  v = b ? ov : v;
+  k = b ? ok : k;
+  p = b ? op : p;
+}
+
+__device__ __forceinline__ int lane_read(int x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
+__device__ __forceinline__ float lane_read(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+__device__ __forceinline__ double lane_read(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Argmin over the 64 lanes of a wavefront (all lanes active).  Four DPP steps leave every
+// lane of each 16-lane row with the row's winner; the four row winners are read into
+// scalar registers and combined.  Every lane ends with the same (wave-uniform) result.
 // (value, key) is a strict total order, so the result does not depend on the order of
 // combination; `p` is a payload carried with the winning key.
 template <typename T>
 __device__ __forceinline__ void wave_argmin(T& v, int& k, int& p) {
+  dpp_step<kDppXor1>(v, k, p);
+  dpp_step<kDppXor2>(v, k, p);
+  dpp_step<kDppHalfMirror>(v, k, p);
+  dpp_step<kDppMirror>(v, k, p);
+  T bv = lane_read(v, 0);
+  int bk = lane_read(k, 0), bp = lane_read(p, 0);
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const T ov = __shfl_xor(v, off);
-    const int ok = __shfl_xor(k, off);
-    const int op = __shfl_xor(p, off);
-    if (better(ov, ok, v, k)) { v = ov; k = ok; p = op; }
+  for (int row = 1; row < 4; ++row) {
+    const T rv = lane_read(v, 16 * row);
+    const int rk = lane_read(k, 16 * row), rp = lane_read(p, 16 * row);
+    const bool b = better(rv, rk, bv, bk);
+    bv = b ? rv : bv;
+    bk = b ? rk : bk;
+    bp = b ? rp : bp;
   }
+  v = bv;
+  k = bk;
+  p = bp;
 }
 
 template <typename T>
@@ -74,10 +143,19 @@ struct Reducer {
       __syncthreads();
       v = s[0].v; k = s[0].k; p = s[0].p;
 #pragma unroll
-      for (int q = 1; q < NW; ++q)
-        if (better(s[q].v, s[q].k, v, k)) { v = s[q].v; k = s[q].k; p = s[q].p; }
+      for (int q = 1; q < NW; ++q) {
+        const T qv = s[q].v;
+        const int qk = s[q].k, qp = s[q].p;
+        const bool b = better(qv, qk, v, k);
+        v = b ? qv : v;
+        k = b ? qk : k;
+        p = b ? qp : p;
+      }
       phase ^= 1;
     }
+    v = uniform(v);
+    k = uniform(k);
+    p = uniform(p);
   }
 };
 
@@ -107,98 +185,111 @@ __device__ __forceinline__ T init_entry(int i, int j, int m, int n, const double
   return T(0);
 }
 
-// simplex iterations on objective row `orow`:
-// 0 optimal, 1 unbounded (s_out = entering column), 2 iteration cap, 3 numeric trouble
-template <typename T, class Tab>
-__device__ int simplex_loop(Tab& t, int orow, const Opt<T>& o, int maxit, int& it, int* p1,
-                            int& s_out) {
-  int deg = 0;
-  for (;;) {
-    if (it >= maxit) return 2;
-    const bool bland = deg >= o.bland_after;
-    const int s = t.pricing(orow, bland, o.eps_d);
-    if (s < 0) return 0;
-    T th;
-    const int r = t.ratio(s, bland, o.eps_p, th);
-    if (r < 0) { s_out = s; return orow == 0 ? 1 : 3; }
-    t.pivot(r, s);
-    ++it;
-    if (p1) ++*p1;
-    deg = (th == T(0)) ? deg + 1 : 0;
-  }
-}
+// Phase I (equality pivots, then one auxiliary column) and phase II, written as a state
+// machine so that every backend's pivot / pricing / ratio code is inlined exactly once.
+// Statement for statement the same decisions, in the same order, as the oracle's
+// solve_impl (oracle/simplex_core.inc), which is what makes the results bit-identical.
+enum : int { kStCrash = 0, kStEq = 1, kStAux = 2, kStP1 = 3, kStP1End = 4, kStP2 = 5 };
 
-// Phase I (equality pivots, then one auxiliary column) and phase II.  Mirrors the
-// oracle's solve_impl statement by statement.
+// `warm` (optional, m entries): a target basis (variable ids, -1 = none).  Before phase I
+// its structurals are pivoted in one by one (crash), each on the row of largest |T[i][s]|
+// among rows whose basic variable is not itself a target; a structural with no eligible
+// nonzero is skipped (its row keeps the slack).  Phase I / II then repair and finish.
 template <typename T, class Tab>
-__device__ void run_simplex(Tab& t, int m, int n, const int8_t* sense, const Opt<T>& o,
-                            int& status, int& it, int& p1, int& unb, T& wfin) {
+__device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* sense,
+                                            const Opt<T>& o, int& status, int& it, int& p1,
+                                            int& unb, T& wfin, const int32_t* warm = nullptr) {
   const int maxit = o.max_iter > 0 ? o.max_iter : 50 * (m + n) + 1000;
   const int C = n + 2;
   it = 0; p1 = 0; status = kStatusOptimal; unb = -1; wfin = T(0);
-
-  if (sense) {
-    for (int i = 0; i < m && status == kStatusOptimal; ++i) {
-      if (sense[i] != 1) continue;
-      const int r = 2 + i;
+  int stage = warm ? kStCrash : (sense ? kStEq : kStAux);
+  int ei = 0, deg = 0, ci = 0;
+  for (;;) {
+    int pr = -1, ps = -1;
+    bool kill_after = false, count_p1 = false;
+    if (stage == kStCrash) {
+      int v = -1;
+      while (ci < m && v < 0) {
+        const int w = warm[ci];
+        ++ci;
+        if (w >= 0 && w < n) v = w;
+      }
+      if (v < 0) { stage = sense ? kStEq : kStAux; continue; }
+      const int s = t.find_nonbasic(v);
+      if (s < 0) continue;
+      if (it >= maxit) { status = kStatusIterLimit; break; }
+      const int r = t.absmax_row(s, o.eps_p);
+      if (r < 0) continue;
+      const int lv = t.bs_of(r);
+      pr = r; ps = s; count_p1 = true;
+      kill_after = lv >= n && lv < n + m && sense && sense[lv - n] == 1;
+    } else if (stage == kStEq) {
+      while (ei < m && sense[ei] != 1) ++ei;
+      if (ei >= m) { stage = kStAux; continue; }
+      const int r = 2 + ei;
+      ++ei;
+      if (t.bs_of(r) != n + r - 2) continue;  // its slack already left (crash)
       if (it >= maxit) { status = kStatusIterLimit; break; }
       const int j = t.absmax_col(r, o.eps_p);
       if (j < 0) {
         const T bb = t.get(r, 0);
-        if (tabs(bb) <= o.tol) t.mark_redund(r);
-        else status = kStatusInfeasible;
+        if (tabs(bb) <= o.tol) { t.mark_redund(r); continue; }
+        status = kStatusInfeasible;
+        break;
+      }
+      pr = r; ps = j; kill_after = true; count_p1 = true;
+    } else if (stage == kStAux) {
+      const int rmin = t.min_rhs_row();
+      deg = 0;
+      if (rmin >= 0 && t.get(rmin, 0) < -o.tol) {
+        t.setup_x0();
+        if (it >= maxit) { status = kStatusIterLimit; break; }
+        pr = rmin; ps = C - 1; count_p1 = true;
+        stage = kStP1;
+      } else {
+        stage = kStP2;
         continue;
       }
-      t.pivot(r, j);
-      t.kill_col(j);
-      ++it; ++p1;
-    }
-  }
-
-  if (status == kStatusOptimal) {
-    const int rmin = t.min_rhs_row();
-    if (rmin >= 0 && t.get(rmin, 0) < -o.tol) {
-      t.setup_x0();
-      if (it >= maxit) {
-        status = kStatusIterLimit;
+    } else if (stage == kStP1 || stage == kStP2) {
+      if (it >= maxit) { status = kStatusIterLimit; break; }
+      const bool bland = deg >= o.bland_after;
+      const int s = t.pricing(stage == kStP1 ? 1 : 0, bland, o.eps_d);
+      if (s < 0) {
+        if (stage == kStP1) { stage = kStP1End; continue; }
+        status = kStatusOptimal;
+        break;
+      }
+      T th;
+      const int r = t.ratio(s, bland, o.eps_p, th);
+      if (r < 0) {
+        if (stage == kStP2) { status = kStatusUnbounded; unb = t.nb_of(s); }
+        else status = kStatusNumeric;
+        break;
+      }
+      pr = r; ps = s; count_p1 = stage == kStP1;
+      deg = (th == T(0)) ? deg + 1 : 0;
+    } else {  // kStP1End
+      wfin = t.get(1, 0);
+      if (wfin < -o.tol) { status = kStatusInfeasible; break; }
+      const int rx = t.find_basic(n + m);
+      stage = kStP2;
+      deg = 0;
+      if (rx >= 0) {
+        const int j = t.absmax_col(rx, o.eps_p);
+        t.set(rx, 0, T(0));
+        if (j < 0) { t.mark_redund(rx); continue; }
+        if (it >= maxit) { status = kStatusIterLimit; break; }
+        pr = rx; ps = j; kill_after = true; count_p1 = true;
       } else {
-        t.pivot(rmin, C - 1);
-        ++it; ++p1;
-        int sd = -1;
-        const int rc = simplex_loop(t, 1, o, maxit, it, &p1, sd);
-        if (rc == 2) status = kStatusIterLimit;
-        else if (rc == 3) status = kStatusNumeric;
-        else {
-          wfin = t.get(1, 0);
-          if (wfin < -o.tol) {
-            status = kStatusInfeasible;
-          } else {
-            const int rx = t.find_basic(n + m);
-            if (rx >= 0) {
-              const int j = t.absmax_col(rx, o.eps_p);
-              t.set(rx, 0, T(0));
-              if (j >= 0) {
-                if (it >= maxit) status = kStatusIterLimit;
-                else { t.pivot(rx, j); t.kill_col(j); ++it; ++p1; }
-              } else {
-                t.mark_redund(rx);
-              }
-            } else {
-              const int j = t.find_nonbasic(n + m);
-              if (j >= 0) t.kill_col(j);
-            }
-          }
-        }
+        const int j = t.find_nonbasic(n + m);
+        if (j >= 0) t.kill_col(j);
+        continue;
       }
     }
-  }
-
-  if (status == kStatusOptimal) {
-    int s_unb = -1;
-    const int rc = simplex_loop(t, 0, o, maxit, it, (int*)nullptr, s_unb);
-    if (rc == 1) { status = kStatusUnbounded; unb = t.nb_of(s_unb); }
-    else if (rc == 2) status = kStatusIterLimit;
-    else if (rc == 3) status = kStatusNumeric;
+    t.pivot(pr, ps);
+    ++it;
+    if (count_p1) ++p1;
+    if (kill_after) t.kill_col(ps);
   }
 }
 
