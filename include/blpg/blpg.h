@@ -63,6 +63,8 @@ typedef struct {
   const double *c;     /* n, maximised */
   const int8_t *sense; /* m: 0 '<=', 1 '=', 2 '>='; NULL = all '<=' */
   int32_t family;      /* parametric family id for the learned warm start; -1 none */
+  const int32_t *warm_basis; /* m variable ids to crash in before phase I (id < n structural,
+                                n <= id < n+m slack of row id-n; -1 skip), or NULL (cold) */
 } blpg_lp;
 
 /* Per-LP result.  Replaces Simplex::getResultAll / getStatusAll (simplex.cu:424-461),
@@ -131,8 +133,8 @@ typedef struct {
   int32_t rcap[16], ccap[16];
   int64_t count[16];
   double kernel_ms[16];
-  double verify_ms;
-  double fallback_ms;
+  double verify_ms;     /* longest FP64 check of one bin */
+  double fallback_ms;   /* longest FP64 fallback (+ re-check) of one bin */
   int64_t fallback_count;
   int64_t total_pivots;
   double total_ms;      /* event time from first to last GPU operation of the solve */
@@ -140,7 +142,14 @@ typedef struct {
 int blpg_batch_info_get(blpg_ctx *ctx, const blpg_batch *batch, blpg_batch_info *info);
 
 /* Seeded synthetic LP generators (counter-based; bit-identical on every host).
- * config 1..5 as documented in DESIGN.md; writes `count` LPs [first, first+count) of a
+ * config 1: A ~ U[0,1) 30% zeroed, b ~ U[1,10), c ~ U[0,1);
+ * config 2: A ~ U[-1,1) 40% zeroed, b ~ U[0,10) with floor(0.1m) zero rows, c ~ N(0,1),
+ *           floor(0.05 batch) LPs with one unbounded column;
+ * config 3: A ~ U[-1,1) 50% zeroed, b ~ N(0,5), c ~ N(0,1), round(0.3m) '=' rows,
+ *           floor(0.03 batch) LPs with a contradictory row pair;
+ * config 4: parametric family `shape_idx` (A_f fixed; b, c scaled by 1 + 0.05 N(0,1));
+ * config 5: 27x32 as config 1, other shapes as config 2 without planted columns.
+ * Writes `count` LPs [first, first+count) of a
  * batch of `batch_count` LPs with shape (m, n): A (count*m*n), b (count*m),
  * c (count*n), sense (count*m, may be NULL for configs without senses),
  * flags (count; bit0 planted unbounded, bit1 planted infeasible). */

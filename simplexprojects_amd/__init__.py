@@ -40,7 +40,8 @@ class BlpgError(RuntimeError):
 class _LP(C.Structure):
     _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("A", C.POINTER(C.c_double)),
                 ("b", C.POINTER(C.c_double)), ("c", C.POINTER(C.c_double)),
-                ("sense", C.POINTER(C.c_int8)), ("family", C.c_int32)]
+                ("sense", C.POINTER(C.c_int8)), ("family", C.c_int32),
+                ("warm_basis", C.POINTER(C.c_int32))]
 
 
 class _Result(C.Structure):
@@ -122,6 +123,7 @@ class LP:
     sense: Optional[np.ndarray] = None
     family: int = -1
     flags: int = 0
+    warm_basis: Optional[np.ndarray] = None
 
     def __post_init__(self):
         self.A = np.ascontiguousarray(self.A, dtype=np.float64)
@@ -133,6 +135,10 @@ class LP:
             self.sense = np.ascontiguousarray(self.sense, dtype=np.int8).reshape(-1)
             if self.sense.size != self.b.size:
                 raise ValueError("sense must have m entries")
+        if self.warm_basis is not None:
+            self.warm_basis = np.ascontiguousarray(self.warm_basis, dtype=np.int32).reshape(-1)
+            if self.warm_basis.size != self.b.size:
+                raise ValueError("warm_basis must have m entries")
 
     @property
     def m(self) -> int:
@@ -153,6 +159,8 @@ def _lp_struct(lp: LP) -> _LP:
     s.A, s.b, s.c = _dp(lp.A), _dp(lp.b), _dp(lp.c)
     s.sense = lp.sense.ctypes.data_as(C.POINTER(C.c_int8)) if lp.sense is not None else None
     s.family = lp.family
+    s.warm_basis = (lp.warm_basis.ctypes.data_as(C.POINTER(C.c_int32))
+                    if lp.warm_basis is not None else None)
     return s
 
 

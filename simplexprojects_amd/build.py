@@ -35,6 +35,7 @@ CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROOT / 'include'}"]
 
 HIP_SOURCES = ["kernel_stream.hip", "kernel_tile.hip", "kernel_verify.hip", "kernel_util.hip",
                "kernel_mlp.hip", "abi.cpp", "simplex_shim.cpp", "warmstart.cpp", "serve.cpp"]
+SHIM_TEST = PKG / "test_shim"  # C++ drop-in test program (tests/cpp/test_shim.cpp)
 CXX_SOURCES = ["gen.cpp"]
 
 
@@ -58,12 +59,6 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build_oracle() -> Path:
-    """Compile the CPU oracle (test infrastructure) with its Makefile."""
-    _run(["make", "-s", "-C", str(ROOT / "oracle")])
-    return ROOT / "oracle" / "liboracle.so"
-
-
 def build(verbose: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
     sources = [CSRC / s for s in HIP_SOURCES + CXX_SOURCES if (CSRC / s).exists()]
@@ -74,6 +69,9 @@ def build(verbose: bool = False) -> Path:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
           *map(str, objs), "-lpthread"])
     shutil.move(str(tmp), str(LIB))
+    # the C++ drop-in test program links the library like a reference caller would
+    _run(["g++", "-O1", "-std=c++17", f"-I{ROOT / 'include'}", str(ROOT / "tests/cpp/test_shim.cpp"),
+          str(LIB), "-Wl,-rpath,$ORIGIN", "-o", str(SHIM_TEST)])
     if verbose:
         print(f"built {LIB}")
     return LIB
@@ -81,4 +79,3 @@ def build(verbose: bool = False) -> Path:
 
 if __name__ == "__main__":
     build(verbose=True)
-    build_oracle()

@@ -70,12 +70,13 @@ struct blpg_batch {
   Out out{};
   Check chk{};
   int32_t* d_fb = nullptr;        // fallback-used flag per LP
+  int32_t* d_warm = nullptr;      // warm-start bases (basis pool layout), or nullptr
   int32_t* d_all = nullptr;       // 0..count-1
   int32_t* d_fail = nullptr;      // LP ids failing the check
   int32_t* d_counters = nullptr;  // [0..nbins) queues, then verify/fallback queues and counts
   std::vector<Bin> bins;
   blpg_batch_info info{};
-  hipEvent_t ev[40] = {};
+  hipEvent_t ev[96] = {};
   int nev = 0;
 };
 
@@ -319,7 +320,11 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   const size_t o_f64 = off; off += al(sizeof(double) * n * 5);      // obj wfin chkobj kktp kktd
   const size_t o_x = off; off += al(sizeof(double) * B->x_elems * 2);
   const size_t o_basis = off; off += al(sizeof(int32_t) * B->basis_elems);
-  const size_t o_cnt = off; off += al(sizeof(int32_t) * 64);
+  const size_t o_cnt = off; off += al(sizeof(int32_t) * 128);
+  bool any_warm = false;
+  for (int64_t k = 0; k < count; ++k) any_warm |= lps[k].warm_basis != nullptr;
+  const size_t o_warm = off;
+  if (any_warm) off += al(sizeof(int32_t) * B->basis_elems);
   B->mem_bytes = off;
   if (ctx->opt.hbm_budget > 0 && (int64_t)off > ctx->opt.hbm_budget) {
     delete B;
@@ -355,6 +360,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->chk.x = B->out.x + B->x_elems;
   B->out.basis = (int32_t*)(base + o_basis);
   B->d_counters = (int32_t*)(base + o_cnt);
+  if (any_warm) B->d_warm = (int32_t*)(base + o_warm);
 
   // ---- host staging (pinned) and one upload ----
   const size_t stage_bytes = o_i32 + sizeof(int32_t) * n * 9;
@@ -385,12 +391,22 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   hipError_t e = hipMemcpyAsync(B->mem, h, stage_bytes, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipHostFree(h);
+  if (e == hipSuccess && any_warm) {
+    std::vector<int32_t> w(B->basis_elems, -1);
+    for (int64_t k = 0; k < count; ++k)
+      if (lps[k].warm_basis)
+        for (int i = 0; i < lps[k].m; ++i) {
+          const int32_t v = lps[k].warm_basis[i];
+          w[B->desc[k].basis + i] = (v >= 0 && v < lps[k].n + lps[k].m) ? v : -1;
+        }
+    e = hipMemcpy(B->d_warm, w.data(), sizeof(int32_t) * w.size(), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     (void)hipFree(B->mem);
     delete B;
     return fail(ctx, BLPG_E_HIP, "batch upload failed: %s", hipGetErrorString(e));
   }
-  for (int i = 0; i < 40; ++i)
+  for (int i = 0; i < 96; ++i)
     if (hipEventCreate(&B->ev[i]) != hipSuccess) {
       blpg_batch_free(ctx, B);
       return fail(ctx, BLPG_E_HIP, "hipEventCreate failed");
@@ -409,7 +425,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
 void blpg_batch_free(blpg_ctx* ctx, blpg_batch* B) {
   if (!B) return;
   if (ctx) (void)hipSetDevice(ctx->device);
-  for (int i = 0; i < 40; ++i)
+  for (int i = 0; i < 96; ++i)
     if (B->ev[i]) (void)hipEventDestroy(B->ev[i]);
   if (B->mem) (void)hipFree(B->mem);
   delete B;
@@ -482,6 +498,10 @@ static int launch_bin(blpg_ctx* ctx, blpg_batch* B, const Bin& b, const BinPlan&
   return BLPG_OK;
 }
 
+// Event slots of a batch: ev[0] start; per bin i: ev[1+4i] solve start, ev[2+4i] solve end,
+// ev[3+4i] verify end, ev[4+4i] fallback end; ev[65] end of everything.
+static constexpr int kEvEnd = 65;
+
 int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
   if (!ctx || !B) return BLPG_E_INVALID;
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -491,99 +511,113 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
   B->precision = precision;
   int32_t* q = B->d_counters;
   const int nb = (int)B->bins.size();
-
-  // ---- plans and one HBM workspace carved per bin ----
-  std::vector<BinPlan> plans(nb);
-  std::vector<size_t> ws_off(nb, 0);
-  size_t ws_total = 0;
-  for (int i = 0; i < nb; ++i) {
-    const int rc = plan_bin(ctx, B->bins[i], precision, (int64_t)B->bins[i].ids.size(), plans[i]);
-    if (rc) return rc;
-    ws_off[i] = ws_total;
-    ws_total += plans[i].ws_bytes;
-  }
-  if (ws_total) {
-    const int rc = grow(ctx, ctx->ws, ws_total);
-    if (rc) return rc;
-  }
-
-  HIPCHK(ctx, hipEventRecord(B->ev[0], st));
-  HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
-  // ---- fork: every size bin on its own stream, concurrently ----
-  HIPCHK(ctx, hipEventRecord(ctx->fork, st));
-  for (int i = 0; i < nb; ++i) {
-    const Bin& b = B->bins[i];
-    hipStream_t s = nb == 1 ? st : ctx->aux[i % blpg_ctx::kAux];
-    if (s != st) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->fork, 0));
-    HIPCHK(ctx, hipEventRecord(B->ev[1 + 2 * i], s));
-    const int rc = launch_bin(ctx, B, b, plans[i], precision, b.d_idx, (int32_t)b.ids.size(),
-                              nullptr, q + i, (char*)ctx->ws.p + ws_off[i], nullptr, s);
-    if (rc) return rc;
-    HIPCHK(ctx, hipEventRecord(B->ev[2 + 2 * i], s));
-    if (s != st) HIPCHK(ctx, hipStreamWaitEvent(st, B->ev[2 + 2 * i], 0));
-  }
-  B->nev = 1 + 2 * nb;
-  HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));  // all solve kernels done
-  // ---- FP64 check of the final bases, and the FP64 fallback for FP32 failures ----
   double eps_d, eps_p, tol, kkt;
   resolve(ctx->opt, eps_d, eps_p, tol, kkt);
   const bool do_verify = ctx->opt.verify || precision == BLPG_FP32;
-  if (do_verify) {
-    VerifyLaunch V{};
-    V.desc = B->d_desc; V.pool = B->d_pool; V.spool = B->d_spool;
-    V.idx = B->d_all; V.count = (int32_t)B->count;
-    V.out = B->out; V.chk = B->chk; V.tol = kkt;
-    V.fp32_source = precision == BLPG_FP32;
-    V.mcap = B->mmax;
-    V.ncap = B->nmax;
-    V.grid = persistent_grid(ctx, verify_max_blocks_per_cu(B->mmax, B->nmax, true), B->count);
-    const int64_t kcap = std::min(B->mmax, B->nmax);
-    V.ws_stride = (kcap * kcap + 63) & ~63ll;
-    {
-      const int rc = grow(ctx, ctx->vws, sizeof(double) * (size_t)V.ws_stride * V.grid);
-      if (rc) return rc;
-      V.ws = (double*)ctx->vws.p;
+  const bool fp32 = precision == BLPG_FP32;
+
+  // ---- launch plans; HBM workspaces carved per bin (bins run concurrently) ----
+  std::vector<BinPlan> plans(nb), fplans(nb);
+  std::vector<Bin> fbins(nb);
+  std::vector<size_t> ws_off(nb, 0), fws_off(nb, 0), vws_off(nb, 0);
+  std::vector<int> vgrid(nb, 1);
+  std::vector<int64_t> vstride(nb, 0);
+  size_t ws_total = 0, fws_total = 0, vws_total = 0;
+  for (int i = 0; i < nb; ++i) {
+    const Bin& b = B->bins[i];
+    const int64_t cnt = (int64_t)b.ids.size();
+    int rc = plan_bin(ctx, b, precision, cnt, plans[i]);
+    if (rc) return rc;
+    ws_off[i] = ws_total;
+    ws_total += plans[i].ws_bytes;
+    if (do_verify) {
+      const int mc = b.rcap - 2, nc = b.ccap - 2;
+      vgrid[i] = persistent_grid(ctx, verify_max_blocks_per_cu(mc, nc, true), cnt);
+      const int64_t kcap = std::min(mc, nc);
+      vstride[i] = (kcap * kcap + 63) & ~63ll;
+      vws_off[i] = vws_total;
+      vws_total += sizeof(double) * (size_t)vstride[i] * vgrid[i];
     }
-    V.queue = q + 20;
-    V.fail_list = B->d_fail;
-    V.fail_count = q + 21;
-    HIPCHK(ctx, launch_fill_i32(q + 20, 0, 2, st));
-    HIPCHK(ctx, launch_verify(V, st));
-    HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));  // end of verify
-    if (precision == BLPG_FP32) {
-      // FP64 re-solve of every LP whose FP32 result failed the check, warm-started from
-      // its FP32 basis (crash); the LP count is read on the device (q[21]), so no host
-      // synchronisation is needed.
-      Bin fb;
-      fb.rcap = B->mmax + 2;
-      fb.ccap = B->nmax + 2;
-      fb.nt = 256;
-      const int tv = Tuner::best_tile(B->mmax, B->nmax, BLPG_FP64);
-      if (tv >= 0) { fb.kernel = BLPG_KERNEL_TILE; fb.variant = tv; }
-      else if (stream_lds_bytes(BLPG_FP64, true, fb.nt, fb.rcap, fb.ccap) <= ctx->tuner.lds_limit())
-        fb.kernel = BLPG_KERNEL_LDS;
-      else fb.kernel = BLPG_KERNEL_STREAM;
-      BinPlan FP;
-      int rc = plan_bin(ctx, fb, BLPG_FP64, B->count, FP);
+    if (fp32) {
+      // FP64 fallback kernel of this bin
+      Bin& f = fbins[i];
+      f.rcap = b.rcap; f.ccap = b.ccap; f.nt = Tuner::stream_threads(b.rcap - 2, b.ccap - 2);
+      const int tv = Tuner::best_tile(b.rcap - 2, b.ccap - 2, BLPG_FP64);
+      if (tv >= 0) { f.kernel = BLPG_KERNEL_TILE; f.variant = tv; }
+      else if (stream_lds_bytes(BLPG_FP64, true, f.nt, f.rcap, f.ccap) <= ctx->tuner.lds_limit() &&
+               stream_max_blocks_per_cu(BLPG_FP64, true, f.nt, f.rcap, f.ccap) > 0)
+        f.kernel = BLPG_KERNEL_LDS;
+      else f.kernel = BLPG_KERNEL_STREAM;
+      rc = plan_bin(ctx, f, BLPG_FP64, cnt, fplans[i]);
       if (rc) return rc;
-      if (FP.ws_bytes) { rc = grow(ctx, ctx->ws2, FP.ws_bytes); if (rc) return rc; }
-      rc = launch_bin(ctx, B, fb, FP, BLPG_FP64, B->d_fail, -1, q + 21, q + 22, ctx->ws2.p,
-                      B->out.basis, st);
-      if (rc) return rc;
-      // re-check the FP64 results
-      VerifyLaunch V2 = V;
-      V2.idx = B->d_fail;
-      V2.count = -1;
-      V2.count_dev = q + 21;
-      V2.fp32_source = 0;
-      V2.queue = q + 23;
-      V2.fail_list = nullptr;
-      V2.fail_count = nullptr;
-      HIPCHK(ctx, launch_fill_i32(q + 23, 0, 1, st));
-      HIPCHK(ctx, launch_verify(V2, st));
+      fws_off[i] = fws_total;
+      fws_total += fplans[i].ws_bytes;
     }
-    HIPCHK(ctx, hipEventRecord(B->ev[B->nev++], st));  // end of fallback
   }
+  if (ws_total) { const int rc = grow(ctx, ctx->ws, ws_total); if (rc) return rc; }
+  if (fws_total) { const int rc = grow(ctx, ctx->ws2, fws_total); if (rc) return rc; }
+  if (vws_total) { const int rc = grow(ctx, ctx->vws, vws_total); if (rc) return rc; }
+
+  HIPCHK(ctx, hipEventRecord(B->ev[0], st));
+  HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
+  HIPCHK(ctx, launch_fill_i32(q, 0, 128, st));
+  // ---- fork: each size bin runs solve -> check -> fallback on its own stream ----
+  HIPCHK(ctx, hipEventRecord(ctx->fork, st));
+  int64_t pos = 0;
+  for (int i = 0; i < nb; ++i) {
+    const Bin& b = B->bins[i];
+    const int32_t cnt = (int32_t)b.ids.size();
+    int32_t* qb = q + 8 * i;  // [0] solve queue [1] verify queue [2] fail count
+                              // [3] fallback queue [4] re-verify queue
+    hipStream_t s = nb == 1 ? st : ctx->aux[i % blpg_ctx::kAux];
+    if (s != st) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->fork, 0));
+    HIPCHK(ctx, hipEventRecord(B->ev[1 + 4 * i], s));
+    int rc = launch_bin(ctx, B, b, plans[i], precision, b.d_idx, cnt, nullptr, qb,
+                        (char*)ctx->ws.p + ws_off[i], B->d_warm, s);
+    if (rc) return rc;
+    HIPCHK(ctx, hipEventRecord(B->ev[2 + 4 * i], s));
+    if (do_verify) {
+      VerifyLaunch V{};
+      V.desc = B->d_desc; V.pool = B->d_pool; V.spool = B->d_spool;
+      V.idx = b.d_idx; V.count = cnt;
+      V.out = B->out; V.chk = B->chk; V.tol = kkt;
+      V.fp32_source = fp32;
+      V.mcap = b.rcap - 2;
+      V.ncap = b.ccap - 2;
+      V.grid = vgrid[i];
+      V.ws_stride = vstride[i];
+      V.ws = (double*)((char*)ctx->vws.p + vws_off[i]);
+      V.queue = qb + 1;
+      V.fail_list = B->d_fail + pos;
+      V.fail_count = qb + 2;
+      HIPCHK(ctx, launch_verify(V, s));
+      HIPCHK(ctx, hipEventRecord(B->ev[3 + 4 * i], s));
+      if (fp32) {
+        // FP64 re-solve of the LPs of this bin whose FP32 result failed; the count is
+        // read on the device, so no host synchronisation is needed.
+        rc = launch_bin(ctx, B, fbins[i], fplans[i], BLPG_FP64, B->d_fail + pos, -1, qb + 2,
+                        qb + 3, (char*)ctx->ws2.p + fws_off[i], B->out.basis, s);
+        if (rc) return rc;
+        VerifyLaunch V2 = V;
+        V2.idx = B->d_fail + pos;
+        V2.count = -1;
+        V2.count_dev = qb + 2;
+        V2.fp32_source = 0;
+        V2.queue = qb + 4;
+        V2.fail_list = nullptr;
+        V2.fail_count = nullptr;
+        HIPCHK(ctx, launch_verify(V2, s));
+      }
+      HIPCHK(ctx, hipEventRecord(B->ev[4 + 4 * i], s));
+    }
+    if (s != st) {
+      HIPCHK(ctx, hipEventRecord(B->ev[2 + 4 * i + (do_verify ? 2 : 0)], s));
+      HIPCHK(ctx, hipStreamWaitEvent(st, B->ev[2 + 4 * i + (do_verify ? 2 : 0)], 0));
+    }
+    pos += cnt;
+  }
+  HIPCHK(ctx, hipEventRecord(B->ev[kEvEnd], st));
+  B->nev = kEvEnd + 1;
   return BLPG_OK;
 }
 
@@ -597,20 +631,20 @@ int blpg_batch_info_get(blpg_ctx* ctx, const blpg_batch* B, blpg_batch_info* inf
   if (!ctx || !B || !info) return BLPG_E_INVALID;
   *info = B->info;
   const int nb = (int)B->bins.size();
+  const bool checked = ctx->opt.verify || B->precision == BLPG_FP32;
   float ms = 0;
-  for (int i = 0; i < nb; ++i) {
-    if (hipEventElapsedTime(&ms, B->ev[1 + 2 * i], B->ev[2 + 2 * i]) == hipSuccess)
-      info->kernel_ms[i] = ms;
-  }
-  const int e_solve = 1 + 2 * nb;
   info->verify_ms = info->fallback_ms = 0;
-  if (B->nev > e_solve + 1 &&
-      hipEventElapsedTime(&ms, B->ev[e_solve], B->ev[e_solve + 1]) == hipSuccess)
-    info->verify_ms = ms;
-  if (B->nev > e_solve + 2 &&
-      hipEventElapsedTime(&ms, B->ev[e_solve + 1], B->ev[e_solve + 2]) == hipSuccess)
-    info->fallback_ms = ms;
-  if (B->nev > 0 && hipEventElapsedTime(&ms, B->ev[0], B->ev[B->nev - 1]) == hipSuccess)
+  for (int i = 0; i < nb; ++i) {
+    if (hipEventElapsedTime(&ms, B->ev[1 + 4 * i], B->ev[2 + 4 * i]) == hipSuccess)
+      info->kernel_ms[i] = ms;
+    if (checked) {
+      if (hipEventElapsedTime(&ms, B->ev[2 + 4 * i], B->ev[3 + 4 * i]) == hipSuccess)
+        info->verify_ms = std::max(info->verify_ms, (double)ms);
+      if (hipEventElapsedTime(&ms, B->ev[3 + 4 * i], B->ev[4 + 4 * i]) == hipSuccess)
+        info->fallback_ms = std::max(info->fallback_ms, (double)ms);
+    }
+  }
+  if (B->nev > 0 && hipEventElapsedTime(&ms, B->ev[0], B->ev[kEvEnd]) == hipSuccess)
     info->total_ms = ms;
   return BLPG_OK;
 }
@@ -687,6 +721,7 @@ int blpg_solve_shared_A(blpg_ctx* ctx, int32_t m, int32_t n, const double* A, co
     lps[k].c = C + k * (int64_t)n;
     lps[k].sense = sense;
     lps[k].family = -1;
+    lps[k].warm_basis = nullptr;
   }
   return blpg_solve_batch(ctx, lps.data(), count, out);
 }

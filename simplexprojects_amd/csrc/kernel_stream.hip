@@ -104,12 +104,13 @@ struct StreamTab {
       if (i >= 2 && a > eps_p) {
         const T beta = at(i, 0);
         const T th = (beta > T(0) ? beta : T(0)) / a;
+        const T v = th > T(0) ? th : -a;  // degenerate rows: largest pivot first
         const int k = bland ? bs[i] : i;
-        if (better(th, k, bv, bk)) { bv = th; bk = k; bp = i; }
+        if (better(v, k, bv, bk)) { bv = v; bk = k; bp = i; }
       }
     }
     red.argmin(bv, bk, bp);
-    theta = bv;
+    theta = bv > T(0) ? bv : T(0);
     return bp;
   }
 

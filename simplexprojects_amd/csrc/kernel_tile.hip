@@ -165,12 +165,13 @@ struct TileTab {
       if (a > eps_p) {
         const T beta = rhsbuf[i];
         const T th = (beta > T(0) ? beta : T(0)) / a;
+        const T v = th > T(0) ? th : -a;  // degenerate rows: largest pivot first
         const int key = bland ? bs[i] : i;
-        if (better(th, key, bv, bk)) { bv = th; bk = key; bp = i; }
+        if (better(v, key, bv, bk)) { bv = v; bk = key; bp = i; }
       }
     }
     red.argmin(bv, bk, bp);
-    theta = bv;
+    theta = bv > T(0) ? bv : T(0);
     return bp;
   }
 

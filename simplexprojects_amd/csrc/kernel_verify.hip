@@ -346,6 +346,12 @@ __global__ __launch_bounds__(64) void k_verify(const LpDesc* __restrict__ desc,
         chk.fb[lp] = 1;
       }
     }
+    // The fallback is warm-started from the FP32 basis only when that basis was
+    // (nearly) optimal; other failed results restart cold (crash list cleared).
+    if (fp32_source && ok == 0 && fail_list && status != kStatusOptimal) {
+      int32_t* bw = out.basis + D.basis;
+      for (int i = lane; i < m; i += 64) bw[i] = -1;
+    }
   }
 }
 

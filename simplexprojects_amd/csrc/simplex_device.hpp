@@ -9,7 +9,9 @@
 // Rule (the build's own; the reference's is simplex.cu:93-376):
 //   pricing  s = argmin (T[orow][j], j) over live columns with T[orow][j] < -eps_d
 //            (reference: most negative reduced cost incl. 0, simplex.cu:123-143)
-//   ratio    r = argmin (max(T[i][0],0) / T[i][s], i) over rows with T[i][s] > eps_p
+//   ratio    r = argmin (key_i, i) over rows with T[i][s] > eps_p, key_i = theta_i =
+//            max(T[i][0],0) / T[i][s] when positive, else -T[i][s] (degenerate rows rank
+//            by the largest pivot element)
 //            (reference: simplex.cu:186-236, tolerance 1e-10, ">=" ties)
 //   no ratio candidate in phase II = unbounded (the reference exits silently, :268)
 //   Bland's rule after `bland_after` consecutive degenerate pivots (the reference has

@@ -1,0 +1,125 @@
+"""GPU edge cases of the drop-in path: empty and tiny batches, degenerate / cycling LPs,
+equality handling, the iteration cap, invalid input, sizes beyond the register tile, and
+the C++ drop-in program."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import simplexprojects_amd as sa
+from tests.common import HALL_MCKINNON, HAND_LPS
+
+pytestmark = pytest.mark.gpu
+
+
+def test_empty_batch():
+    S = sa.Solver()
+    r = S.solve([])
+    assert len(r) == 0
+    S.close()
+
+
+def test_one_by_one():
+    S = sa.Solver()
+    r = S.solve([sa.LP([[2.0]], [4.0], [3.0]), sa.LP([[1.0]], [1.0], [-1.0]),
+                 sa.LP([[-1.0]], [1.0], [1.0])])
+    assert list(r.status) == [sa.OPTIMAL, sa.OPTIMAL, sa.UNBOUNDED]
+    assert r.objective[0] == 6.0 and r.objective[1] == 0.0
+    S.close()
+
+
+@pytest.mark.parametrize("kernel", ["tile", "stream"])
+def test_hand_lps(kernel):
+    S = sa.Solver(kernel=kernel)
+    names = sorted(HAND_LPS)
+    r = S.solve([sa.LP(HAND_LPS[n][0], HAND_LPS[n][1], HAND_LPS[n][2]) for n in names])
+    for k, n in enumerate(names):
+        assert r.status[k] == sa.OPTIMAL and abs(r.objective[k] - HAND_LPS[n][3]) < 1e-12
+    S.close()
+
+
+def test_cycling_needs_bland():
+    A, b, c = HALL_MCKINNON
+    S = sa.Solver(bland_after=10**9, max_iter=300)
+    assert S.solve([sa.LP(A, b, c)]).status[0] == sa.ITER_LIMIT
+    S.close()
+    S = sa.Solver()
+    r = S.solve([sa.LP(A, b, c), sa.LP(np.vstack([A, np.ones(4)]), [0, 0, 1.0], c)])
+    assert r.status[0] == sa.UNBOUNDED and r.status[1] == sa.OPTIMAL
+    assert abs(r.objective[1] - 0.875) < 1e-12
+    S.close()
+
+
+def test_equalities_redundant_and_contradictory():
+    # x1 + x2 = 2 twice (redundant), max x1: 2
+    A = np.array([[1.0, 1.0], [1.0, 1.0]])
+    lp1 = sa.LP(A, [2.0, 2.0], [1.0, 0.0], sense=[1, 1])
+    # x1 + x2 = 2 and x1 + x2 = 3: infeasible
+    lp2 = sa.LP(A, [2.0, 3.0], [1.0, 0.0], sense=[1, 1])
+    # all-zero equality row with b != 0: infeasible in stage 1
+    lp3 = sa.LP(np.array([[0.0, 0.0], [1.0, 1.0]]), [1.0, 5.0], [1.0, 1.0], sense=[1, 0])
+    # '>=' row: x1 + x2 >= 3, x1 <= 1, min x2 (max -x2): x2 = 2
+    lp4 = sa.LP(np.array([[1.0, 1.0], [1.0, 0.0]]), [3.0, 1.0], [0.0, -1.0], sense=[2, 0])
+    S = sa.Solver()
+    r = S.solve([lp1, lp2, lp3, lp4])
+    assert r.status[0] == sa.OPTIMAL and abs(r.objective[0] - 2) < 1e-12
+    assert r.status[1] == sa.INFEASIBLE and r.status[2] == sa.INFEASIBLE
+    assert r.status[3] == sa.OPTIMAL and abs(r.objective[3] + 2) < 1e-12
+    for k, lp in enumerate([lp1, lp2, lp3, lp4]):
+        assert r.status[k] == oracle.solve(lp.A, lp.b, lp.c, lp.sense)["status"]
+    S.close()
+
+
+def test_iteration_cap():
+    lp = sa.gen_config(2, 2, 105, 103, 1, shape_idx=3, batch_count=1024)[0]
+    S = sa.Solver(max_iter=3)
+    r = S.solve([lp])
+    assert r.status[0] == sa.ITER_LIMIT and r.pivots[0] == 3
+    S.close()
+
+
+def test_invalid_input_rejected():
+    S = sa.Solver()
+    A = np.ones((2, 2))
+    A[0, 1] = np.nan
+    with pytest.raises(sa.BlpgError) as e:
+        S.solve([sa.LP(A, [1, 1], [1, 1])])
+    assert e.value.code == sa.E_INVALID
+    with pytest.raises(sa.BlpgError):
+        S.solve([sa.LP(np.ones((1, 1)), [1.0], [1.0], sense=[7])])
+    S.close()
+
+
+def test_large_lp_uses_streamed_kernel():
+    # 300 x 280 exceeds every register-tile variant and the LDS: streamed from HBM
+    rng = np.random.default_rng(5)
+    lps = [sa.LP(rng.uniform(0, 1, (300, 280)) * (rng.uniform(size=(300, 280)) < 0.3),
+                 rng.uniform(1, 10, 300), rng.uniform(0, 1, 280)) for _ in range(4)]
+    S = sa.Solver(verify=False)
+    b = S.upload(lps)
+    b.solve()
+    b.sync()
+    assert b.info()["bins"][0]["kernel"] == sa.KERNEL_STREAM
+    r = b.download()
+    for k, lp in enumerate(lps):
+        o = oracle.solve(lp.A, lp.b, lp.c)
+        assert r.pivots[k] == o["pivots"] and r.objective[k] == o["objective"]
+    S.close()
+
+
+def test_hbm_budget_rejects():
+    S = sa.Solver(hbm_budget=1024)
+    with pytest.raises(sa.BlpgError) as e:
+        S.solve(sa.gen_config(1, 0, 27, 32, 4))
+    assert e.value.code == sa.E_BUDGET
+    S.close()
+
+
+def test_cpp_drop_in_program():
+    exe = os.path.join(os.path.dirname(sa.LIB_PATH), "test_shim")
+    assert os.path.exists(exe), "built by simplexprojects_amd.build"
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout
