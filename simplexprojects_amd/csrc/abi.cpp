@@ -279,33 +279,48 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
     B->nmax = std::max(B->nmax, L.n);
   }
 
-  // ---- size bins, routed by the tuner ----
-  std::map<std::pair<int, int>, int> binof;  // (kernel/variant key, cap key) -> bin
-  for (int64_t k = 0; k < count; ++k) {
-    const Route r = ctx->tuner.route(B->desc[k].m, B->desc[k].n, ctx->opt.precision,
-                                     ctx->opt.kernel);
-    const std::pair<int, int> key{r.kernel * 1000 + r.variant, r.nt * 100000 + r.rcap * 0 + r.bin};
-    auto it = binof.find(key);
-    int bi;
-    if (it == binof.end()) {
-      bi = (int)B->bins.size();
-      binof[key] = bi;
-      Bin b;
-      b.kernel = r.kernel;
-      b.variant = r.variant;
-      b.nt = r.nt;
-      B->bins.push_back(b);
-    } else {
-      bi = it->second;
+  // ---- size bins, routed by the tuner: one bin per kernel instance (a tile variant or a
+  //      streamed kernel handles every LP up to its capacity); LDS bins are split by size
+  //      class and folded into the HBM-streamed kernel if the batch would need > 16 bins ----
+  std::vector<Route> routes(count);
+  for (int64_t k = 0; k < count; ++k)
+    routes[k] = ctx->tuner.route(B->desc[k].m, B->desc[k].n, ctx->opt.precision, ctx->opt.kernel);
+  auto bin_key = [](const Route& r) -> long {
+    if (r.kernel == BLPG_KERNEL_TILE) return 1000000L + r.variant;
+    if (r.kernel == BLPG_KERNEL_STREAM) return 2000000L + r.nt;
+    return 3000000L + (long)r.nt * 100000L + r.bin;  // LDS: per size class
+  };
+  for (int pass = 0; pass < 2; ++pass) {
+    std::map<long, int> binof;
+    B->bins.clear();
+    for (int64_t k = 0; k < count; ++k) {
+      const Route& r = routes[k];
+      const long key = bin_key(r);
+      auto it = binof.find(key);
+      int bi;
+      if (it == binof.end()) {
+        bi = (int)B->bins.size();
+        binof[key] = bi;
+        Bin b;
+        b.kernel = r.kernel;
+        b.variant = r.variant;
+        b.nt = r.nt;
+        B->bins.push_back(b);
+      } else {
+        bi = it->second;
+      }
+      Bin& b = B->bins[bi];
+      b.ids.push_back((int32_t)k);
+      b.rcap = std::max(b.rcap, B->desc[k].m + 2);
+      b.ccap = std::max(b.ccap, B->desc[k].n + 2);
     }
-    Bin& b = B->bins[bi];
-    b.ids.push_back((int32_t)k);
-    b.rcap = std::max(b.rcap, B->desc[k].m + 2);
-    b.ccap = std::max(b.ccap, B->desc[k].n + 2);
+    if (B->bins.size() <= 16) break;
+    for (auto& r : routes)
+      if (r.kernel == BLPG_KERNEL_LDS) r.kernel = BLPG_KERNEL_STREAM;
   }
   if (B->bins.size() > 16) {
     delete B;
-    return fail(ctx, BLPG_E_INVALID, "more than 16 size bins in one batch");
+    return fail(ctx, BLPG_E_INVALID, "batch needs more than 16 kernel instances");
   }
 
   // ---- one device allocation, carved ----
