@@ -37,6 +37,10 @@ PER_SHAPE = 1024
 SEED = 2
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
+# same counters on a plain 256 MiB device copy (FETCH x2.000, WRITE x1.000); the commands and
+# raw counts are in profiles/tile_fp32_105x103_mixed_batch.md.
+PROFILED_TRAFFIC = {("tile", "fp32", 105, 103, 1024): 92.65e6}
 
 
 def build_batch(sa):
@@ -114,7 +118,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: LPs are independent, so the batch shards "
-              "across GPUs with no exchange step (see DESIGN.md); run with --gpus 1.",
+              "across GPUs with no exchange step; run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -165,7 +169,8 @@ def main():
                                "to host",
                    "lps": count, "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": PROFILED_TRAFFIC.get((kname, "fp32") + bin_shape + (len(idx),)),
                      "kernel": f"{kname} fp32 bin {bin_shape[0]}x{bin_shape[1]}",
                      "launch_ms": ms, "algorithmic_bytes": bytes_launch,
                      "fp64_valu_equiv_tflops": flops_launch / (ms * 1e-3) / 1e12},
