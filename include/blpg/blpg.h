@@ -125,6 +125,20 @@ int blpg_batch_sync(blpg_ctx *ctx);
 int blpg_batch_download(blpg_ctx *ctx, blpg_batch *batch, blpg_result *out);
 void blpg_batch_free(blpg_ctx *ctx, blpg_batch *batch);
 
+/* Bulk results of the last blpg_batch_solve without per-LP copies: waits for the solve,
+ * copies the batch's results with one DMA into host memory the batch owns (pinned) and
+ * points `res` at it.  The arrays stay valid until the next solve, fetch or free of the
+ * batch.  LP k's x is x[x_offset[k] .. + n_k), its basis basis[basis_offset[k] .. + m_k). */
+typedef struct {
+  int64_t count;
+  const int32_t *status, *pivots, *phase1_pivots, *fallback_used;
+  const double *objective, *kkt_primal, *kkt_dual;
+  const double *x;
+  const int32_t *basis;
+  const int64_t *x_offset, *basis_offset;
+} blpg_batch_results;
+int blpg_batch_fetch(blpg_ctx *ctx, blpg_batch *batch, blpg_batch_results *res);
+
 /* Timing and routing of the last blpg_batch_solve, for the benchmark:
  * kernel_ms[k] = HIP-event time of the solve kernels of bin k (on the stream they run on). */
 typedef struct {
@@ -138,8 +152,73 @@ typedef struct {
   int64_t fallback_count;
   int64_t total_pivots;
   double total_ms;      /* event time from first to last GPU operation of the solve */
+  double warmstart_ms;  /* learned warm start inference (features, network, basis pick) */
+  int32_t param[16];    /* per bin: tile variant (TILE) or threads per workgroup */
 } blpg_batch_info;
 int blpg_batch_info_get(blpg_ctx *ctx, const blpg_batch *batch, blpg_batch_info *info);
+
+/* ---- measured autotuning (no reference counterpart: the reference fixes one block size
+ * per launch, simplex.cu:558) ----
+ * blpg_autotune times, for every LP shape in `lps`, each kernel instance that can hold it
+ * (register-tile variants, LDS-resident and HBM-streamed tableaux at 64..512 threads) on
+ * those LPs in the context's precision (best of `reps` solves after one warm-up), and
+ * routes later batches of that shape and precision to the fastest.  The choice changes
+ * speed only: all instances compute bit-identical results.  blpg_tune_set pins a route by
+ * hand (param: tile variant, or threads per workgroup); export / import carry the table as
+ * text lines "precision m n kernel param ms". */
+int blpg_autotune(blpg_ctx *ctx, const blpg_lp *lps, int64_t count, int32_t reps);
+int blpg_tune_set(blpg_ctx *ctx, int32_t precision, int32_t m, int32_t n, int32_t kernel,
+                  int32_t param);
+int blpg_tune_export(blpg_ctx *ctx, char *buf, int64_t cap, int64_t *len);
+int blpg_tune_import(blpg_ctx *ctx, const char *text);
+
+/* ---- learned warm start (no reference counterpart) ----
+ * A per-family network maps the normalised (b, c) of an LP of a parametric family (fixed
+ * A and shape, varying b and c) to one logit per variable (structurals 0..n-1, then the
+ * slacks n..n+m-1): x -> hidden (ReLU) -> hidden (ReLU) -> n+m, bf16 weights and
+ * activations on MFMA with FP32 accumulation and FP32 master weights (Adam).  It is
+ * trained on the GPU from optimal bases the solver itself produced.  At solve time the m
+ * variables with the largest logits form the crash basis (most confident first); the crash
+ * and its repair (dual simplex when the crashed basis is dual feasible, else the auxiliary
+ * phase I) run inside the solve kernels, so a wrong prediction costs pivots, never the
+ * answer. */
+typedef struct blpg_ws_model blpg_ws_model;
+typedef struct {
+  int32_t hidden;      /* hidden width, a multiple of 64; 0 = 256 */
+  int32_t steps;       /* full-batch training steps; 0 = 300 */
+  float lr;            /* Adam learning rate; 0 = 1e-3 */
+  float weight_decay;  /* decoupled weight decay; default 0 */
+  uint64_t seed;       /* weight initialisation */
+} blpg_ws_config;
+
+int blpg_ws_create(blpg_ctx *ctx, int32_t families, int32_t m, int32_t n,
+                   const blpg_ws_config *cfg, blpg_ws_model **model);
+void blpg_ws_destroy(blpg_ctx *ctx, blpg_ws_model *model);
+/* Trains on `count` LPs (lps[k].family in [0, families), shape m x n) whose optimal bases
+ * (m variable ids each) are `bases`; fits the per-family input normalisation first.
+ * loss (optional, cfg.steps entries): mean over families of the cross-entropy per step. */
+int blpg_ws_train(blpg_ctx *ctx, blpg_ws_model *model, const blpg_lp *lps,
+                  const int32_t *bases, int64_t count, float *loss);
+/* Logits (count x (n+m), optional) and warm-start lists (count x m, optional). */
+int blpg_ws_predict(blpg_ctx *ctx, blpg_ws_model *model, const blpg_lp *lps, int64_t count,
+                    float *logits, int32_t *warm);
+/* Attach (or detach with NULL) a model to a device batch: every later blpg_batch_solve
+ * evaluates it on the batch's LPs with family >= 0 and the model's shape and crashes
+ * from its prediction. */
+int blpg_batch_set_warmstart(blpg_ctx *ctx, blpg_batch *batch, blpg_ws_model *model);
+/* FP32 master parameters of one family, unpadded: W1 (hidden x (m+n)), b1, W2 (hidden x
+ * hidden), b2, W3 ((n+m) x hidden), b3.  `get` / `set` may be NULL; *count receives the
+ * number of floats. */
+int blpg_ws_params(blpg_ctx *ctx, blpg_ws_model *model, int32_t family, float *get,
+                   const float *set, int64_t *count);
+/* Per-family input normalisation (mu, 1/sd over the m+n inputs b then c). */
+int blpg_ws_norm(blpg_ctx *ctx, const blpg_ws_model *model, int32_t family, float *mu,
+                 float *inv_sd);
+/* One forward + backward pass without an update (gradient check): fits the normalisation
+ * to `lps` like blpg_ws_train, then writes grad (families x count floats, the
+ * blpg_ws_params layout) and loss (families entries). */
+int blpg_ws_gradient(blpg_ctx *ctx, blpg_ws_model *model, const blpg_lp *lps,
+                     const int32_t *bases, int64_t count, float *grad, float *loss);
 
 /* Seeded synthetic LP generators (counter-based; bit-identical on every host).
  * config 1: A ~ U[0,1) 30% zeroed, b ~ U[1,10), c ~ U[0,1);

@@ -64,7 +64,22 @@ class _Info(C.Structure):
                 ("ccap", C.c_int32 * 16), ("count", C.c_int64 * 16),
                 ("kernel_ms", C.c_double * 16), ("verify_ms", C.c_double),
                 ("fallback_ms", C.c_double), ("fallback_count", C.c_int64),
-                ("total_pivots", C.c_int64), ("total_ms", C.c_double)]
+                ("total_pivots", C.c_int64), ("total_ms", C.c_double),
+                ("warmstart_ms", C.c_double), ("param", C.c_int32 * 16)]
+
+
+class _BatchResults(C.Structure):
+    _fields_ = [("count", C.c_int64), ("status", C.POINTER(C.c_int32)),
+                ("pivots", C.POINTER(C.c_int32)), ("phase1_pivots", C.POINTER(C.c_int32)),
+                ("fallback_used", C.POINTER(C.c_int32)), ("objective", C.POINTER(C.c_double)),
+                ("kkt_primal", C.POINTER(C.c_double)), ("kkt_dual", C.POINTER(C.c_double)),
+                ("x", C.POINTER(C.c_double)), ("basis", C.POINTER(C.c_int32)),
+                ("x_offset", C.POINTER(C.c_int64)), ("basis_offset", C.POINTER(C.c_int64))]
+
+
+class _WsConfig(C.Structure):
+    _fields_ = [("hidden", C.c_int32), ("steps", C.c_int32), ("lr", C.c_float),
+                ("weight_decay", C.c_float), ("seed", C.c_uint64)]
 
 
 # every symbol include/blpg/blpg.h declares (checked by the CPU test suite)
@@ -72,7 +87,10 @@ ABI_SYMBOLS = [
     "blpg_default_options", "blpg_create", "blpg_destroy", "blpg_last_error",
     "blpg_set_options", "blpg_solve_batch", "blpg_solve_shared_A", "blpg_batch_upload",
     "blpg_batch_solve", "blpg_batch_sync", "blpg_batch_download", "blpg_batch_free",
-    "blpg_batch_info_get", "blpg_gen_config", "blpg_version",
+    "blpg_batch_fetch", "blpg_autotune", "blpg_tune_set", "blpg_tune_export", "blpg_tune_import",
+    "blpg_batch_info_get", "blpg_gen_config", "blpg_version", "blpg_ws_create",
+    "blpg_ws_destroy", "blpg_ws_train", "blpg_ws_predict", "blpg_batch_set_warmstart",
+    "blpg_ws_params", "blpg_ws_norm", "blpg_ws_gradient",
 ]
 
 _lib: Optional[C.CDLL] = None
@@ -103,11 +121,27 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
     lib.blpg_batch_sync.argtypes = [C.c_void_p]
     lib.blpg_batch_download.argtypes = [C.c_void_p, C.c_void_p, P(_Result)]
     lib.blpg_batch_free.argtypes = [C.c_void_p, C.c_void_p]
+    lib.blpg_batch_fetch.argtypes = [C.c_void_p, C.c_void_p, P(_BatchResults)]
+    lib.blpg_autotune.argtypes = [C.c_void_p, P(_LP), C.c_int64, C.c_int32]
+    lib.blpg_tune_set.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_int32]
+    lib.blpg_tune_export.argtypes = [C.c_void_p, C.c_char_p, C.c_int64, P(C.c_int64)]
+    lib.blpg_tune_import.argtypes = [C.c_void_p, C.c_char_p]
     lib.blpg_batch_info_get.argtypes = [C.c_void_p, C.c_void_p, P(_Info)]
     lib.blpg_gen_config.argtypes = [C.c_int32, C.c_uint64, C.c_int32, C.c_int64, C.c_int64,
                                     C.c_int64, C.c_int32, C.c_int32, P(C.c_double),
                                     P(C.c_double), P(C.c_double), P(C.c_int8), P(C.c_int32)]
     lib.blpg_version.restype = C.c_char_p
+    F, I32 = P(C.c_float), P(C.c_int32)
+    lib.blpg_ws_create.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(_WsConfig),
+                                   P(C.c_void_p)]
+    lib.blpg_ws_destroy.argtypes = [C.c_void_p, C.c_void_p]
+    lib.blpg_ws_train.argtypes = [C.c_void_p, C.c_void_p, P(_LP), I32, C.c_int64, F]
+    lib.blpg_ws_predict.argtypes = [C.c_void_p, C.c_void_p, P(_LP), C.c_int64, F, I32]
+    lib.blpg_batch_set_warmstart.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.blpg_ws_params.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, F, F, P(C.c_int64)]
+    lib.blpg_ws_norm.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, F, F]
+    lib.blpg_ws_gradient.argtypes = [C.c_void_p, C.c_void_p, P(_LP), I32, C.c_int64, F, F]
     if path is None:
         _lib = lib
     return lib
@@ -245,17 +279,48 @@ class Batch:
         self.solver._check(self.solver.lib.blpg_batch_download(self.solver.ctx, self.h, res))
         return _collect(res, xs, bs)
 
+    def fetch_raw(self) -> _BatchResults:
+        """blpg_batch_fetch: results in the batch's pinned host buffers (no per-LP copies)."""
+        r = _BatchResults()
+        self.solver._check(self.solver.lib.blpg_batch_fetch(self.solver.ctx, self.h, C.byref(r)))
+        return r
+
+    def fetch(self) -> Results:
+        r = self.fetch_raw()
+        k = r.count
+        if k == 0:
+            z = np.zeros(0, np.int32)
+            return Results(z, z, z, z, np.zeros(0), np.zeros(0), np.zeros(0), [], [])
+        arr = lambda p, n: np.ctypeslib.as_array(p, shape=(n,)).copy()
+        nx = sum(lp.n for lp in self.lps)
+        nb = sum(lp.m for lp in self.lps)
+        xo = arr(r.x_offset, k)
+        bo = arr(r.basis_offset, k)
+        xpool, bpool = arr(r.x, nx), arr(r.basis, nb)
+        xs = [xpool[xo[i]:xo[i] + lp.n] for i, lp in enumerate(self.lps)]
+        bs = [bpool[bo[i]:bo[i] + lp.m] for i, lp in enumerate(self.lps)]
+        return Results(arr(r.status, k), arr(r.pivots, k), arr(r.phase1_pivots, k),
+                       arr(r.fallback_used, k), arr(r.objective, k), arr(r.kkt_primal, k),
+                       arr(r.kkt_dual, k), xs, bs)
+
     def info(self) -> dict:
         inf = _Info()
         self.solver._check(self.solver.lib.blpg_batch_info_get(self.solver.ctx, self.h,
                                                                C.byref(inf)))
         nb = inf.nbins
         return {"bins": [{"kernel": inf.kernel[i], "rcap": inf.rcap[i], "ccap": inf.ccap[i],
-                          "count": inf.count[i], "kernel_ms": inf.kernel_ms[i]}
+                          "count": inf.count[i], "kernel_ms": inf.kernel_ms[i],
+                          "param": inf.param[i]}
                          for i in range(nb)],
                 "verify_ms": inf.verify_ms, "fallback_ms": inf.fallback_ms,
                 "fallback_count": inf.fallback_count, "total_pivots": inf.total_pivots,
-                "total_ms": inf.total_ms}
+                "total_ms": inf.total_ms, "warmstart_ms": inf.warmstart_ms}
+
+    def set_warmstart(self, model: Optional["WarmStart"]) -> None:
+        """Crash the batch's family LPs from `model`'s prediction on every later solve()."""
+        self.model = model
+        self.solver._check(self.solver.lib.blpg_batch_set_warmstart(
+            self.solver.ctx, self.h, model.h if model is not None else None))
 
     def free(self) -> None:
         if self.h:
@@ -330,10 +395,132 @@ class Solver:
     def upload(self, lps: Sequence[LP]) -> Batch:
         return Batch(self, lps)
 
+    def autotune(self, lps: Sequence[LP], reps: int = 3) -> list[tuple]:
+        """Measure every kernel instance per LP shape and route to the fastest; returns
+        the tuning table as (precision, m, n, kernel, param, ms) tuples."""
+        lps = list(lps)
+        arr = (_LP * len(lps))(*[_lp_struct(lp) for lp in lps])
+        self._check(self.lib.blpg_autotune(self.ctx, arr, len(lps), reps))
+        return self.tune_table()
+
+    def tune_set(self, m: int, n: int, kernel: str, param: int, precision: Optional[str] = None):
+        p = self.opts.precision if precision is None else {"fp64": FP64, "fp32": FP32}[precision]
+        self._check(self.lib.blpg_tune_set(self.ctx, p, m, n, _KERNELS[kernel], param))
+
+    def tune_export(self) -> str:
+        n = C.c_int64()
+        self._check(self.lib.blpg_tune_export(self.ctx, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        self._check(self.lib.blpg_tune_export(self.ctx, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
+    def tune_import(self, text: str) -> None:
+        self._check(self.lib.blpg_tune_import(self.ctx, text.encode()))
+
+    def tune_table(self) -> list[tuple]:
+        rows = []
+        for line in self.tune_export().splitlines():
+            p, m, n, k, q, ms = line.split()
+            rows.append((int(p), int(m), int(n), int(k), int(q), float(ms)))
+        return rows
+
     def close(self) -> None:
         if getattr(self, "ctx", None):
             self.lib.blpg_destroy(self.ctx)
             self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f32p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _i32p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class WarmStart:
+    """Learned warm start (blpg_ws_*): one network per parametric family of shape m x n,
+    trained on the GPU from optimal bases, evaluated on the GPU at solve time."""
+
+    def __init__(self, solver: Solver, families: int, m: int, n: int, hidden: int = 256,
+                 steps: int = 300, lr: float = 1e-3, weight_decay: float = 0.0, seed: int = 0):
+        self.solver, self.families, self.m, self.n = solver, families, m, n
+        cfg = _WsConfig(hidden, steps, lr, weight_decay, seed)
+        self.steps = steps
+        h = C.c_void_p()
+        solver._check(solver.lib.blpg_ws_create(solver.ctx, families, m, n, C.byref(cfg),
+                                                C.byref(h)))
+        self.h = h
+        cnt = C.c_int64()
+        solver._check(solver.lib.blpg_ws_params(solver.ctx, h, 0, None, None, C.byref(cnt)))
+        self.n_params = cnt.value
+
+    def _lps(self, lps):
+        lps = list(lps)
+        return lps, (_LP * len(lps))(*[_lp_struct(lp) for lp in lps])
+
+    def train(self, lps: Sequence[LP], bases: Sequence[np.ndarray]) -> np.ndarray:
+        lps, arr = self._lps(lps)
+        B = np.ascontiguousarray(np.stack(bases), dtype=np.int32)
+        loss = np.zeros(self.steps, dtype=np.float32)
+        self.solver._check(self.solver.lib.blpg_ws_train(self.solver.ctx, self.h, arr, _i32p(B),
+                                                         len(lps), _f32p(loss)))
+        return loss
+
+    def predict(self, lps: Sequence[LP]):
+        lps, arr = self._lps(lps)
+        logits = np.zeros((len(lps), self.n + self.m), dtype=np.float32)
+        warm = np.zeros((len(lps), self.m), dtype=np.int32)
+        self.solver._check(self.solver.lib.blpg_ws_predict(self.solver.ctx, self.h, arr, len(lps),
+                                                           _f32p(logits), _i32p(warm)))
+        return logits, warm
+
+    def params(self, family: int, set: Optional[np.ndarray] = None) -> np.ndarray:
+        out = np.zeros(self.n_params, dtype=np.float32)
+        src = None if set is None else np.ascontiguousarray(set, dtype=np.float32)
+        self.solver._check(self.solver.lib.blpg_ws_params(self.solver.ctx, self.h, family,
+                                                          _f32p(out), _f32p(src), None))
+        return out
+
+    def norm(self, family: int):
+        d = self.m + self.n
+        mu = np.zeros(d, dtype=np.float32); isd = np.zeros(d, dtype=np.float32)
+        self.solver._check(self.solver.lib.blpg_ws_norm(self.solver.ctx, self.h, family,
+                                                        _f32p(mu), _f32p(isd)))
+        return mu, isd
+
+    def gradient(self, lps: Sequence[LP], bases: Sequence[np.ndarray]):
+        lps, arr = self._lps(lps)
+        B = np.ascontiguousarray(np.stack(bases), dtype=np.int32)
+        g = np.zeros((self.families, self.n_params), dtype=np.float32)
+        loss = np.zeros(self.families, dtype=np.float32)
+        self.solver._check(self.solver.lib.blpg_ws_gradient(self.solver.ctx, self.h, arr,
+                                                            _i32p(B), len(lps), _f32p(g),
+                                                            _f32p(loss)))
+        return g, loss
+
+    def layout(self):
+        """(name, shape) of the parameter blocks in params() / gradient() order."""
+        H, d = self.params_hidden(), self.m + self.n
+        return [("W1", (H, d)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)),
+                ("W3", (d, H)), ("b3", (d,))]
+
+    def params_hidden(self) -> int:
+        d = self.m + self.n
+        # n_params = H*d + H + H*H + H + d*H + d
+        a, b, c = 1, 2 * d + 2, d - self.n_params
+        return int(round((-b + (b * b - 4 * a * c) ** 0.5) / 2))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.solver.lib.blpg_ws_destroy(self.solver.ctx, self.h)
+            self.h = None
 
     def __del__(self):
         try:

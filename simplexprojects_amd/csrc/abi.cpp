@@ -19,87 +19,13 @@
 #include "../../include/blpg/blpg.h"
 #include "blpg_internal.hpp"
 #include "kernels.hpp"
+#include "ctx.hpp"
 #include "tuner.hpp"
 
 using namespace blpg;
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-struct blpg_ctx {
-  blpg_options opt;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipDeviceProp_t props{};
-  std::string err;
-  DevBuf ws;   // HBM slabs of the streamed kernel (one region per concurrent bin)
-  DevBuf ws2;  // HBM slabs of the fallback kernel
-  DevBuf vws;  // LU workspace of the verify kernel
-  Tuner tuner;
-  static constexpr int kAux = 4;  // size bins run concurrently on these (default priority)
-  hipStream_t aux[kAux] = {};
-  hipEvent_t fork = nullptr;
-};
-
-namespace {
-
-struct Bin {
-  int kernel = BLPG_KERNEL_STREAM;  // BLPG_KERNEL_*
-  int variant = 0;                  // tile variant
-  int nt = 256;                     // stream threads
-  int rcap = 0, ccap = 0;
-  std::vector<int32_t> ids;
-  int32_t* d_idx = nullptr;
-};
-
-}  // namespace
-
-struct blpg_batch {
-  int64_t count = 0;
-  int precision = 0;
-  std::vector<LpDesc> desc;
-  int64_t pool_elems = 0, spool_elems = 0, x_elems = 0, basis_elems = 0;
-  int mmax = 0, nmax = 0;
-  void* mem = nullptr;
-  size_t mem_bytes = 0;
-  LpDesc* d_desc = nullptr;
-  double* d_pool = nullptr;
-  int8_t* d_spool = nullptr;
-  Out out{};
-  Check chk{};
-  int32_t* d_fb = nullptr;        // fallback-used flag per LP
-  int32_t* d_warm = nullptr;      // warm-start bases (basis pool layout), or nullptr
-  int32_t* d_all = nullptr;       // 0..count-1
-  int32_t* d_fail = nullptr;      // LP ids failing the check
-  int32_t* d_counters = nullptr;  // [0..nbins) queues, then verify/fallback queues and counts
-  std::vector<Bin> bins;
-  blpg_batch_info info{};
-  hipEvent_t ev[96] = {};
-  int nev = 0;
-};
-
 // ------------------------------------------------------------------------------------
 namespace {
-
-int fail(blpg_ctx* ctx, int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  if (ctx) ctx->err = buf;
-  return code;
-}
-
-#define HIPCHK(ctx, call)                                                                   \
-  do {                                                                                      \
-    hipError_t e_ = (call);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
-      return fail(ctx, BLPG_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
-                  __FILE__, __LINE__);                                                      \
-  } while (0)
 
 void resolve(const blpg_options& o, double& eps_d, double& eps_p, double& tol, double& kkt) {
   const bool f32 = o.precision == BLPG_FP32;
@@ -211,6 +137,8 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->count = count;
   B->precision = ctx->opt.precision;
   B->desc.resize(count);
+  B->family.resize(count);
+  for (int64_t k = 0; k < count; ++k) B->family[k] = lps ? lps[k].family : -1;
   // pack: distinct A / b / c / sense pointers are stored once (shared-A batches)
   std::map<const void*, int64_t> seen;
   std::vector<std::pair<const double*, int64_t>> dsrc;  // (ptr, len) in pool order
@@ -331,9 +259,9 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   const size_t o_desc = off; off += al(sizeof(LpDesc) * n);
   const size_t o_pool = off; off += al(sizeof(double) * B->pool_elems);
   const size_t o_spool = off; off += al(B->spool_elems + 16);
-  const size_t o_i32 = off; off += al(sizeof(int32_t) * n * 9);     // status pivots p1 unb ok fb all fail idx
-  const size_t o_f64 = off; off += al(sizeof(double) * n * 5);      // obj wfin chkobj kktp kktd
-  const size_t o_x = off; off += al(sizeof(double) * B->x_elems * 2);
+  const size_t o_i32 = off; off += al(sizeof(int32_t) * n * 10);    // status pivots p1 unb ok fb all fail idx big
+  const size_t o_f64 = off; off += al(sizeof(double) * n * 6);      // obj wfin chkobj kktp kktd amax
+  const size_t o_x = off; off += al(sizeof(double) * B->x_elems);
   const size_t o_basis = off; off += al(sizeof(int32_t) * B->basis_elems);
   const size_t o_cnt = off; off += al(sizeof(int32_t) * 128);
   bool any_warm = false;
@@ -365,14 +293,20 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->d_all = i32 + 6 * n;
   B->d_fail = i32 + 7 * n;
   int32_t* d_binidx = i32 + 8 * n;
+  B->d_big = i32 + 9 * n;
   double* f64 = (double*)(base + o_f64);
   B->out.obj = f64;
   B->out.wfin = f64 + n;
   B->chk.obj = f64 + 2 * n;
   B->chk.kkt_p = f64 + 3 * n;
   B->chk.kkt_d = f64 + 4 * n;
+  B->out.amax = f64 + 5 * n;
   B->out.x = (double*)(base + o_x);
-  B->chk.x = B->out.x + B->x_elems;
+  B->res_off = o_i32;
+  B->res_bytes = o_basis + sizeof(int32_t) * B->basis_elems - o_i32;
+  B->x_off.resize(n);
+  B->basis_off.resize(n);
+  for (int64_t k = 0; k < n; ++k) { B->x_off[k] = B->desc[k].x; B->basis_off[k] = B->desc[k].basis; }
   B->out.basis = (int32_t*)(base + o_basis);
   B->d_counters = (int32_t*)(base + o_cnt);
   if (any_warm) B->d_warm = (int32_t*)(base + o_warm);
@@ -432,6 +366,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
     B->info.rcap[i] = B->bins[i].rcap;
     B->info.ccap[i] = B->bins[i].ccap;
     B->info.count[i] = (int64_t)B->bins[i].ids.size();
+    B->info.param[i] = B->bins[i].kernel == BLPG_KERNEL_TILE ? B->bins[i].variant : B->bins[i].nt;
   }
   *out = B;
   return BLPG_OK;
@@ -442,6 +377,10 @@ void blpg_batch_free(blpg_ctx* ctx, blpg_batch* B) {
   if (ctx) (void)hipSetDevice(ctx->device);
   for (int i = 0; i < 96; ++i)
     if (B->ev[i]) (void)hipEventDestroy(B->ev[i]);
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  ws_free_rows(B->ws);
+  if (B->h_res) (void)hipHostFree(B->h_res);
+  if (B->warm_mem) (void)hipFree(B->warm_mem);
   if (B->mem) (void)hipFree(B->mem);
   delete B;
 }
@@ -514,8 +453,10 @@ static int launch_bin(blpg_ctx* ctx, blpg_batch* B, const Bin& b, const BinPlan&
 }
 
 // Event slots of a batch: ev[0] start; per bin i: ev[1+4i] solve start, ev[2+4i] solve end,
-// ev[3+4i] verify end, ev[4+4i] fallback end; ev[65] end of everything.
+// ev[3+4i] verify end, ev[4+4i] fallback end; ev[65] end of everything; ev[66], ev[67]
+// bracket the warm-start inference.
 static constexpr int kEvEnd = 65;
+static constexpr int kEvWsStart = 66, kEvWsEnd = 67;  // learned warm start inference
 
 int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
   if (!ctx || !B) return BLPG_E_INVALID;
@@ -535,7 +476,7 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
   std::vector<BinPlan> plans(nb), fplans(nb);
   std::vector<Bin> fbins(nb);
   std::vector<size_t> ws_off(nb, 0), fws_off(nb, 0), vws_off(nb, 0);
-  std::vector<int> vgrid(nb, 1);
+  std::vector<int> vgrid(nb, 1), vgrid2(nb, 1), klds1(nb, 64), klds2(nb, 64);
   std::vector<int64_t> vstride(nb, 0);
   size_t ws_total = 0, fws_total = 0, vws_total = 0;
   for (int i = 0; i < nb; ++i) {
@@ -546,12 +487,17 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
     ws_off[i] = ws_total;
     ws_total += plans[i].ws_bytes;
     if (do_verify) {
+      // first pass factors bases with k <= 32 in LDS (small footprint, many wavefronts per
+      // CU) and defers the rest to a second pass with k <= 64 in LDS (beyond: HBM slab)
       const int mc = b.rcap - 2, nc = b.ccap - 2;
-      vgrid[i] = persistent_grid(ctx, verify_max_blocks_per_cu(mc, nc, true), cnt);
       const int64_t kcap = std::min(mc, nc);
+      klds1[i] = (int)std::min<int64_t>(32, kcap);
+      klds2[i] = (int)std::min<int64_t>(64, kcap);
+      vgrid[i] = persistent_grid(ctx, verify_max_blocks_per_cu(mc, nc, klds1[i]), cnt);
+      vgrid2[i] = persistent_grid(ctx, verify_max_blocks_per_cu(mc, nc, klds2[i]), cnt);
       vstride[i] = (kcap * kcap + 63) & ~63ll;
       vws_off[i] = vws_total;
-      vws_total += sizeof(double) * (size_t)vstride[i] * vgrid[i];
+      vws_total += sizeof(double) * (size_t)vstride[i] * std::max(vgrid[i], vgrid2[i]);
     }
     if (fp32) {
       // FP64 fallback kernel of this bin
@@ -576,16 +522,44 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
   HIPCHK(ctx, hipEventRecord(B->ev[0], st));
   HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
   HIPCHK(ctx, launch_fill_i32(q, 0, 128, st));
-  // ---- fork: each size bin runs solve -> check -> fallback on its own stream ----
+  // learned warm start: crash lists of the family LPs, before any solve kernel
+  HIPCHK(ctx, hipEventRecord(B->ev[kEvWsStart], st));
+  if (B->model) {
+    const int rc = ws_run_batch(ctx, B, st);
+    if (rc) return rc;
+  }
+  HIPCHK(ctx, hipEventRecord(B->ev[kEvWsEnd], st));
+  // ---- fork: each size bin runs solve -> check -> fallback on one of kAux+1 streams;
+  //      bins are placed longest first on the least loaded stream (cost ~ LPs * R * C) ----
   HIPCHK(ctx, hipEventRecord(ctx->fork, st));
-  int64_t pos = 0;
+  constexpr int kStreams = blpg_ctx::kAux + 1;
+  std::vector<int> order(nb), lane_of(nb, 0);
+  for (int i = 0; i < nb; ++i) order[i] = i;
+  auto cost = [&](int i) {
+    return (double)B->bins[i].ids.size() * B->bins[i].rcap * B->bins[i].ccap;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  double load[kStreams] = {};
+  for (int i : order) {
+    int best = 0;
+    for (int l = 1; l < kStreams; ++l)
+      if (load[l] < load[best]) best = l;
+    lane_of[i] = best;
+    load[best] += cost(i);
+  }
+  std::vector<int64_t> first(nb, 0);
+  for (int i = 1; i < nb; ++i) first[i] = first[i - 1] + (int64_t)B->bins[i - 1].ids.size();
+  bool used[kStreams] = {};
   for (int i = 0; i < nb; ++i) {
     const Bin& b = B->bins[i];
     const int32_t cnt = (int32_t)b.ids.size();
+    const int64_t pos = first[i];
     int32_t* qb = q + 8 * i;  // [0] solve queue [1] verify queue [2] fail count
                               // [3] fallback queue [4] re-verify queue
-    hipStream_t s = nb == 1 ? st : ctx->aux[i % blpg_ctx::kAux];
-    if (s != st) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->fork, 0));
+                              // [5] deferred-check count [6] its queue
+    hipStream_t s = lane_of[i] == 0 ? st : ctx->aux[lane_of[i] - 1];
+    if (s != st && !used[lane_of[i]]) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->fork, 0));
+    used[lane_of[i]] = true;
     HIPCHK(ctx, hipEventRecord(B->ev[1 + 4 * i], s));
     int rc = launch_bin(ctx, B, b, plans[i], precision, b.d_idx, cnt, nullptr, qb,
                         (char*)ctx->ws.p + ws_off[i], B->d_warm, s);
@@ -605,7 +579,26 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
       V.queue = qb + 1;
       V.fail_list = B->d_fail + pos;
       V.fail_count = qb + 2;
+      V.klds = klds1[i];
+      const bool two_pass = klds2[i] > klds1[i];
+      if (two_pass) { V.big_list = B->d_big + pos; V.big_count = qb + 5; }
       HIPCHK(ctx, launch_verify(V, s));
+      if (two_pass) {
+        VerifyLaunch Vb = V;
+        Vb.idx = B->d_big + pos;
+        Vb.count = -1;
+        Vb.count_dev = qb + 5;
+        Vb.queue = qb + 6;
+        Vb.klds = klds2[i];
+        Vb.grid = vgrid2[i];
+        Vb.big_list = nullptr;
+        Vb.big_count = nullptr;
+        HIPCHK(ctx, launch_verify(Vb, s));
+      }
+      V.klds = klds2[i];
+      V.grid = vgrid2[i];
+      V.big_list = nullptr;
+      V.big_count = nullptr;
       HIPCHK(ctx, hipEventRecord(B->ev[3 + 4 * i], s));
       if (fp32) {
         // FP64 re-solve of the LPs of this bin whose FP32 result failed; the count is
@@ -629,7 +622,6 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
       HIPCHK(ctx, hipEventRecord(B->ev[2 + 4 * i + (do_verify ? 2 : 0)], s));
       HIPCHK(ctx, hipStreamWaitEvent(st, B->ev[2 + 4 * i + (do_verify ? 2 : 0)], 0));
     }
-    pos += cnt;
   }
   HIPCHK(ctx, hipEventRecord(B->ev[kEvEnd], st));
   B->nev = kEvEnd + 1;
@@ -661,52 +653,79 @@ int blpg_batch_info_get(blpg_ctx* ctx, const blpg_batch* B, blpg_batch_info* inf
   }
   if (B->nev > 0 && hipEventElapsedTime(&ms, B->ev[0], B->ev[kEvEnd]) == hipSuccess)
     info->total_ms = ms;
+  info->warmstart_ms = 0;
+  if (B->nev > 0 && B->model &&
+      hipEventElapsedTime(&ms, B->ev[kEvWsStart], B->ev[kEvWsEnd]) == hipSuccess)
+    info->warmstart_ms = ms;
+  return BLPG_OK;
+}
+
+int blpg_batch_fetch(blpg_ctx* ctx, blpg_batch* B, blpg_batch_results* R) {
+  if (!ctx || !B || !R) return BLPG_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int64_t n = B->count;
+  std::memset(R, 0, sizeof *R);
+  R->count = n;
+  if (n == 0) return BLPG_OK;
+  if (!B->h_res && hipHostMalloc((void**)&B->h_res, B->res_bytes, hipHostMallocDefault) != hipSuccess) {
+    B->h_res = nullptr;
+    return fail(ctx, BLPG_E_NOMEM, "hipHostMalloc(%zu) for results failed", B->res_bytes);
+  }
+  HIPCHK(ctx, hipMemcpyAsync(B->h_res, (char*)B->mem + B->res_off, B->res_bytes,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  // the host span mirrors the device carve: int32 block, FP64 block, x pool, basis pool
+  int32_t* i32 = (int32_t*)B->h_res;
+  double* f64 = (double*)(B->h_res + ((char*)B->out.obj - (char*)B->out.status));
+  const double* x = (const double*)(B->h_res + ((char*)B->out.x - (char*)B->out.status));
+  const int32_t* basis = (const int32_t*)(B->h_res + ((char*)B->out.basis - (char*)B->out.status));
+  const bool verified = ctx->opt.verify || B->precision == BLPG_FP32;
+  int64_t fbc = 0, piv = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    int32_t& status = i32[k];
+    const int ok = i32[4 * n + k];
+    fbc += i32[5 * n + k];
+    piv += i32[n + k];
+    // objective: the FP64 re-solve's when verified, else the tableau value
+    if (verified && ok == 1) f64[k] = f64[2 * n + k];
+    if (!verified) { f64[3 * n + k] = -1.0; f64[4 * n + k] = -1.0; }
+    if (B->precision == BLPG_FP32 && verified && ok == 0 && status == BLPG_OPTIMAL)
+      status = BLPG_NUMERIC;
+  }
+  B->info.fallback_count = fbc;
+  B->info.total_pivots = piv;
+  R->status = i32;
+  R->pivots = i32 + n;
+  R->phase1_pivots = i32 + 2 * n;
+  R->fallback_used = i32 + 5 * n;
+  R->objective = f64;
+  R->kkt_primal = f64 + 3 * n;
+  R->kkt_dual = f64 + 4 * n;
+  R->x = x;
+  R->basis = basis;
+  R->x_offset = B->x_off.data();
+  R->basis_offset = B->basis_off.data();
   return BLPG_OK;
 }
 
 int blpg_batch_download(blpg_ctx* ctx, blpg_batch* B, blpg_result* out) {
   if (!ctx || !B || (B->count > 0 && !out)) return BLPG_E_INVALID;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  const int64_t n = B->count;
-  if (n == 0) return BLPG_OK;
-  std::vector<int32_t> i32(n * 6);
-  std::vector<double> f64(n * 5);
-  std::vector<double> x(B->x_elems * 2);
-  std::vector<int32_t> basis(B->basis_elems);
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  HIPCHK(ctx, hipMemcpy(i32.data(), B->out.status, sizeof(int32_t) * n * 6, hipMemcpyDeviceToHost));
-  HIPCHK(ctx, hipMemcpy(f64.data(), B->out.obj, sizeof(double) * n * 5, hipMemcpyDeviceToHost));
-  if (B->x_elems)
-    HIPCHK(ctx, hipMemcpy(x.data(), B->out.x, sizeof(double) * B->x_elems * 2, hipMemcpyDeviceToHost));
-  if (B->basis_elems)
-    HIPCHK(ctx, hipMemcpy(basis.data(), B->out.basis, sizeof(int32_t) * B->basis_elems,
-                          hipMemcpyDeviceToHost));
-  const bool verified = ctx->opt.verify || B->precision == BLPG_FP32;
-  int64_t fbc = 0, piv = 0;
-  for (int64_t k = 0; k < n; ++k) {
+  blpg_batch_results R;
+  const int rc = blpg_batch_fetch(ctx, B, &R);
+  if (rc) return rc;
+  for (int64_t k = 0; k < B->count; ++k) {
     blpg_result& r = out[k];
     const LpDesc& d = B->desc[k];
-    r.status = i32[k];
-    r.pivots = i32[n + k];
-    r.phase1_pivots = i32[2 * n + k];
-    const int ok = i32[4 * n + k];
-    r.fallback_used = i32[5 * n + k];
-    fbc += r.fallback_used;
-    piv += r.pivots;
-    const bool good = verified && ok == 1;
-    r.objective = good ? f64[2 * n + k] : f64[k];
-    r.kkt_primal = verified ? f64[3 * n + k] : -1.0;
-    r.kkt_dual = verified ? f64[4 * n + k] : -1.0;
-    if (B->precision == BLPG_FP32 && verified && ok == 0 && r.status == BLPG_OPTIMAL)
-      r.status = BLPG_NUMERIC;
-    if (r.x) {
-      const double* src = (good ? x.data() + B->x_elems : x.data()) + d.x;
-      std::memcpy(r.x, src, sizeof(double) * d.n);
-    }
-    if (r.basis) std::memcpy(r.basis, basis.data() + d.basis, sizeof(int32_t) * d.m);
+    r.status = R.status[k];
+    r.pivots = R.pivots[k];
+    r.phase1_pivots = R.phase1_pivots[k];
+    r.fallback_used = R.fallback_used[k];
+    r.objective = R.objective[k];
+    r.kkt_primal = R.kkt_primal[k];
+    r.kkt_dual = R.kkt_dual[k];
+    if (r.x) std::memcpy(r.x, R.x + d.x, sizeof(double) * d.n);
+    if (r.basis) std::memcpy(r.basis, R.basis + d.basis, sizeof(int32_t) * d.m);
   }
-  B->info.fallback_count = fbc;
-  B->info.total_pivots = piv;
   return BLPG_OK;
 }
 
@@ -720,6 +739,92 @@ int blpg_solve_batch(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_resu
   if (!rc) rc = blpg_batch_sync(ctx);
   if (!rc) rc = blpg_batch_download(ctx, B, out);
   blpg_batch_free(ctx, B);
+  return rc;
+}
+
+int blpg_tune_set(blpg_ctx* ctx, int32_t precision, int32_t m, int32_t n, int32_t kernel,
+                  int32_t param) {
+  if (!ctx || precision < 0 || precision > 1 || m < 1 || n < 1 || kernel < BLPG_KERNEL_STREAM ||
+      kernel > BLPG_KERNEL_TILE)
+    return fail(ctx, BLPG_E_INVALID, "invalid arguments to blpg_tune_set");
+  if (kernel == BLPG_KERNEL_TILE &&
+      (param < 0 || param >= tile_variant_count(precision) || !Tuner::fits_tile(param, m, n, precision)))
+    return fail(ctx, BLPG_E_INVALID, "tile variant %d cannot hold %dx%d", param, m, n);
+  if (kernel != BLPG_KERNEL_TILE && param != 64 && param != 128 && param != 256 && param != 512)
+    return fail(ctx, BLPG_E_INVALID, "threads per workgroup must be 64, 128, 256 or 512");
+  ctx->tuner.set_preference(precision, m, n, kernel, param);
+  return BLPG_OK;
+}
+
+int blpg_tune_export(blpg_ctx* ctx, char* buf, int64_t cap, int64_t* len) {
+  if (!ctx) return BLPG_E_INVALID;
+  const std::string t = ctx->tuner.export_text();
+  if (len) *len = (int64_t)t.size();
+  if (buf && cap > 0) {
+    const size_t k = std::min<size_t>(t.size(), (size_t)cap - 1);
+    std::memcpy(buf, t.data(), k);
+    buf[k] = 0;
+  }
+  return BLPG_OK;
+}
+
+int blpg_tune_import(blpg_ctx* ctx, const char* text) {
+  if (!ctx || !text) return BLPG_E_INVALID;
+  if (!ctx->tuner.import_text(text)) return fail(ctx, BLPG_E_INVALID, "malformed tuning table");
+  return BLPG_OK;
+}
+
+int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps) {
+  if (!ctx || (count > 0 && !lps) || count < 0)
+    return fail(ctx, BLPG_E_INVALID, "invalid arguments to blpg_autotune");
+  if (reps < 1) reps = 3;
+  const int prec = ctx->opt.precision;
+  const blpg_options saved = ctx->opt;
+  ctx->opt.kernel = BLPG_KERNEL_AUTO;
+  std::map<std::pair<int, int>, std::vector<blpg_lp>> groups;
+  for (int64_t k = 0; k < count; ++k) {
+    blpg_lp L = lps[k];
+    L.warm_basis = nullptr;  // time cold solves
+    groups[{L.m, L.n}].push_back(L);
+  }
+  int rc = BLPG_OK;
+  for (auto& g : groups) {
+    const int m = g.first.first, n = g.first.second;
+    // candidates: every instance that can hold the shape and be resident
+    std::vector<std::pair<int, int>> cand;
+    for (int v = 0; v < tile_variant_count(prec); ++v)
+      if (Tuner::fits_tile(v, m, n, prec) && tile_max_blocks_per_cu(prec, v) > 0)
+        cand.push_back({BLPG_KERNEL_TILE, v});
+    for (int nt : {64, 128, 256, 512}) {
+      if (stream_lds_bytes(prec, true, nt, m + 2, n + 2) <= ctx->tuner.lds_limit() &&
+          stream_max_blocks_per_cu(prec, true, nt, m + 2, n + 2) > 0)
+        cand.push_back({BLPG_KERNEL_LDS, nt});
+      if (stream_max_blocks_per_cu(prec, false, nt, m + 2, n + 2) > 0)
+        cand.push_back({BLPG_KERNEL_STREAM, nt});
+    }
+    double best_ms = 1e30;
+    std::pair<int, int> best{-1, -1};
+    for (const auto& c : cand) {
+      ctx->tuner.set_preference(prec, m, n, c.first, c.second);
+      blpg_batch* B = nullptr;
+      rc = blpg_batch_upload(ctx, g.second.data(), (int64_t)g.second.size(), &B);
+      if (rc) break;
+      double t = 1e30;
+      for (int r = 0; r <= reps && !rc; ++r) {
+        rc = blpg_batch_solve(ctx, B);
+        if (!rc) rc = blpg_batch_sync(ctx);
+        blpg_batch_info info;
+        if (!rc) rc = blpg_batch_info_get(ctx, B, &info);
+        if (!rc && r > 0 && info.nbins == 1) t = std::min(t, info.kernel_ms[0]);
+      }
+      blpg_batch_free(ctx, B);
+      if (rc) break;
+      if (t < best_ms) { best_ms = t; best = c; }
+    }
+    if (rc) break;
+    if (best.first >= 0) ctx->tuner.set_preference(prec, m, n, best.first, best.second, best_ms);
+  }
+  ctx->opt = saved;
   return rc;
 }
 

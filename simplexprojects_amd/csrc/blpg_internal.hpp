@@ -23,6 +23,7 @@ struct Out {
   double* wfin;     // phase-I objective at the end of phase I
   double* x;        // pool
   int32_t* basis;   // pool
+  double* amax;     // max |A_ij| of the LP's input (written by the solve kernels at set-up)
 };
 
 template <typename T>
@@ -32,12 +33,12 @@ struct Opt {
   int32_t bland_after;
 };
 
-// Result of the FP64 check of a final basis.
+// Result of the FP64 check of a final basis (a verified result's x, the FP64 basic
+// solution, overwrites Out::x).
 struct Check {
   double* obj;      // FP64 objective c^T x
   double* kkt_p;    // primal residual (relative)
   double* kkt_d;    // dual residual (relative)
-  double* x;        // FP64 x (pool, same offsets as Out::x)
   int32_t* ok;      // 1 verified, 0 failed (needs fallback), 2 not checkable (status kept)
   int32_t* fb;      // set to 1 when an FP32 result failed and went to the FP64 fallback
 };

@@ -1,0 +1,123 @@
+// ctx.hpp -- host-side state behind the C ABI handles (context, device batch) and the
+// error helpers shared by abi.cpp and warmstart.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/blpg/blpg.h"
+#include "blpg_internal.hpp"
+#include "tuner.hpp"
+
+namespace blpg {
+
+// Rows of a batched warm-start network evaluation: rows grouped by family, each family's
+// segment padded to a multiple of 64 rows.
+struct WsRows {
+  int64_t rows = 0;            // padded total
+  int nfam = 0;
+  std::vector<int32_t> seg_off, seg_rows, fam_count;  // per family (host copies)
+  void* mem = nullptr;         // device: row_lp, row_fam, seg_off, seg_rows, fam_count + buffers
+  size_t bytes = 0;
+  int32_t *row_lp = nullptr, *row_fam = nullptr, *d_seg_off = nullptr, *d_seg_rows = nullptr,
+          *d_fam_count = nullptr;
+  uint16_t *X = nullptr, *H1 = nullptr, *H2 = nullptr;  // bf16 activations
+  float* Z = nullptr;                                   // logits
+  float* raw = nullptr;        // raw (b, c) features (host-supplied rows), or nullptr
+  int max_seg = 0;             // largest segment (rows)
+};
+
+struct Bin {
+  int kernel = BLPG_KERNEL_STREAM;  // BLPG_KERNEL_*
+  int variant = 0;                  // tile variant
+  int nt = 256;                     // stream threads
+  int rcap = 0, ccap = 0;
+  std::vector<int32_t> ids;
+  int32_t* d_idx = nullptr;
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace blpg
+
+struct blpg_ctx {
+  blpg_options opt;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t props{};
+  std::string err;
+  blpg::DevBuf ws;   // HBM slabs of the streamed kernel (one region per concurrent bin)
+  blpg::DevBuf ws2;  // HBM slabs of the fallback kernel
+  blpg::DevBuf vws;  // LU workspace of the verify kernel
+  blpg::Tuner tuner;
+  // size bins run concurrently on the context stream and these (default priority); with
+  // the context stream that is 4 streams, one per hardware queue of the process
+  static constexpr int kAux = 3;
+  hipStream_t aux[kAux] = {};
+  hipEvent_t fork = nullptr;
+};
+
+struct blpg_batch {
+  int64_t count = 0;
+  int precision = 0;
+  std::vector<blpg::LpDesc> desc;
+  int64_t pool_elems = 0, spool_elems = 0, x_elems = 0, basis_elems = 0;
+  int mmax = 0, nmax = 0;
+  void* mem = nullptr;
+  size_t mem_bytes = 0;
+  blpg::LpDesc* d_desc = nullptr;
+  double* d_pool = nullptr;
+  int8_t* d_spool = nullptr;
+  blpg::Out out{};
+  blpg::Check chk{};
+  int32_t* d_fb = nullptr;        // fallback-used flag per LP
+  int32_t* d_warm = nullptr;      // warm-start lists (basis pool layout, -1 = none), or nullptr
+  std::vector<int32_t> family;    // per LP, -1 none
+  blpg_ws_model* model = nullptr; // learned warm start attached to this batch
+  blpg::WsRows ws;                // its inference rows and buffers
+  void* warm_mem = nullptr;       // warm lists allocated when a model was attached
+  int32_t* d_all = nullptr;       // 0..count-1
+  int32_t* d_fail = nullptr;      // LP ids failing the check
+  int32_t* d_big = nullptr;       // LP ids whose check needs the larger LDS block
+  int32_t* d_counters = nullptr;  // [0..nbins) queues, then verify/fallback queues and counts
+  std::vector<blpg::Bin> bins;
+  size_t res_off = 0, res_bytes = 0;  // device span of the results (one DMA)
+  char* h_res = nullptr;              // pinned host copy of that span
+  std::vector<int64_t> x_off, basis_off;
+  blpg_batch_info info{};
+  hipEvent_t ev[96] = {};
+  int nev = 0;
+};
+
+
+namespace blpg {
+
+inline int fail(blpg_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+// warm-start inference of a batch's attached model (warmstart.cpp)
+int ws_run_batch(blpg_ctx* ctx, blpg_batch* B, hipStream_t st);
+void ws_free_rows(WsRows& R);
+
+}  // namespace blpg
+
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return blpg::fail(ctx, BLPG_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+  } while (0)

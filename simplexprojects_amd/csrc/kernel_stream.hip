@@ -31,6 +31,7 @@ struct StreamTab {
   uint8_t* dead;
   uint8_t* redund;
   uint8_t* target;  // target[var] = 1: var belongs to the warm-start basis
+  double* s_am;     // per-wave max |A_ij| (NW entries)
   Reducer<T, NW> red;
 
   __device__ __forceinline__ T& at(int i, int j) { return tab[(int64_t)i * ld + j]; }
@@ -41,6 +42,12 @@ struct StreamTab {
       const int i = e / C, j = e - (e / C) * C;
       at(i, j) = init_entry<T>(i, j, m, n, A, b, c, sense);
     }
+    // max |A_ij| in FP64 (the check scales its tolerances by it)
+    double am = 0.0;
+    for (int e = tid; e < m * n; e += NT) am = fmax(am, fabs(A[e]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) am = fmax(am, __shfl_xor(am, off));
+    if ((tid & 63) == 0) s_am[tid >> 6] = am;
     for (int i = tid; i < R; i += NT) { bs[i] = i >= 2 ? n + i - 2 : -1; redund[i] = 0; }
     for (int j = tid; j < C; j += NT) {
       nb[j] = j == 0 ? -1 : (j <= n ? j - 1 : n + m);
@@ -49,15 +56,53 @@ struct StreamTab {
     __syncthreads();
   }
 
-  __device__ __forceinline__ void set_target(const int32_t* warm) {
+  // marks the warm-start targets; false when the list holds no valid variable id
+  __device__ __forceinline__ bool set_target(const int32_t* warm) {
     for (int v = threadIdx.x; v < n + m; v += NT) target[v] = 0;
     __syncthreads();
+    int any = 0;
     if (warm)
       for (int i = threadIdx.x; i < m; i += NT) {
         const int v = warm[i];
-        if (v >= 0 && v < n + m) target[v] = 1;
+        if (v >= 0 && v < n + m) { target[v] = 1; any = 1; }
       }
+    return __syncthreads_or(any) != 0;
+  }
+
+  // dual simplex leaving row: most negative rhs below -tol (Bland: smallest basic id)
+  __device__ __forceinline__ int dual_row(bool bland, T tol) {
     __syncthreads();
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    for (int i = 2 + (int)threadIdx.x; i < R; i += NT) {
+      if (redund[i]) continue;
+      const T beta = at(i, 0);
+      if (!(beta < -tol)) continue;
+      const T key = bland ? T(bs[i]) : beta;
+      if (better(key, i, bv, bk)) { bv = key; bk = i; bp = i; }
+    }
+    red.argmin(bv, bk, bp);
+    return bp;
+  }
+
+  // dual simplex entering column for leaving row r
+  __device__ __forceinline__ int dual_ratio(int r, bool bland, T eps_p, T& theta) {
+    __syncthreads();
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    for (int j = 1 + (int)threadIdx.x; j < C; j += NT) {
+      if (dead[j]) continue;
+      const T a = at(r, j);
+      if (!(a < -eps_p)) continue;
+      const T d = at(0, j);
+      const T th = (d > T(0) ? d : T(0)) / -a;
+      const T key = th > T(0) ? th : a;
+      const int kk = bland ? nb[j] : j;
+      if (better(key, kk, bv, bk)) { bv = key; bk = kk; bp = j; }
+    }
+    red.argmin(bv, bk, bp);
+    theta = bv > T(0) ? bv : T(0);
+    return bp;
   }
 
   // row of largest |T[i][s]| among constraint rows whose basic variable is not a target
@@ -232,6 +277,9 @@ struct StreamTab {
     __syncthreads();
     const int tid = threadIdx.x;
     if (tid == 0) {
+      double am = s_am[0];
+      for (int w = 1; w < NW; ++w) am = fmax(am, s_am[w]);
+      out.amax[lp] = am;
       out.status[lp] = status;
       out.pivots[lp] = it;
       out.p1[lp] = p1;
@@ -283,8 +331,10 @@ __global__ __launch_bounds__(NT) void k_stream(const LpDesc* __restrict__ desc,
                                                const int32_t* __restrict__ warm_pool) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_q;
+  __shared__ double s_am[NT / 64];
   const StreamLayout<T> L(LDS_TAB, NT / 64, rcap, ccap);
   StreamTab<T, NT> t;
+  t.s_am = s_am;
   t.colbuf = (T*)(smem + L.colbuf);
   t.rowbuf = (T*)(smem + L.rowbuf);
   t.bs = (int*)(smem + L.bs);
@@ -307,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_stream(const LpDesc* __restrict__ desc,
     const int8_t* sense = d.s >= 0 ? spool + d.s : nullptr;
     t.init(pool + d.a, pool + d.b, pool + d.c, sense);
     const int32_t* warm = warm_pool ? warm_pool + d.basis : nullptr;
-    t.set_target(warm);
+    if (!t.set_target(warm)) warm = nullptr;
     int status, it, p1, unb;
     T wfin;
     run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, warm);

@@ -46,6 +46,7 @@ struct TileTab {
   int* nb;
   uint8_t* redund;
   uint8_t* target;  // target[var] = 1: var belongs to the warm-start basis
+  double* s_am;     // per-wave max |A_ij| (NW entries)
   Reducer<T, NW> red;
 
   __device__ __forceinline__ int row_of(int k) const { return gr + TR * k; }
@@ -67,21 +68,92 @@ struct TileTab {
       const int j = col_of(l);
       if (j == 0 || j >= C - 1) dead |= 1u << l;
     }
+    // max |A_ij| in FP64 (the check scales its tolerances by it)
+    double am = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const int i = row_of(k);
+#pragma unroll
+      for (int l = 0; l < CPT; ++l) {
+        const int j = col_of(l);
+        if (i >= 2 && i < R && j >= 1 && j <= n) am = fmax(am, fabs(A[(int64_t)(i - 2) * n + (j - 1)]));
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) am = fmax(am, __shfl_xor(am, off));
+    if ((tid & 63) == 0) s_am[tid >> 6] = am;
     for (int i = tid; i < RCAP; i += NT) { bs[i] = (i >= 2 && i < R) ? n + i - 2 : -1; redund[i] = 0; }
     for (int j = tid; j < CCAP; j += NT) nb[j] = (j == 0 || j >= C) ? -1 : (j <= n ? j - 1 : n + m);
     col_ready = -1;
     __syncthreads();
   }
 
-  __device__ __forceinline__ void set_target(const int32_t* warm) {
+  // marks the warm-start targets; false when the list holds no valid variable id
+  __device__ __forceinline__ bool set_target(const int32_t* warm) {
     for (int v = tid; v < RCAP + CCAP; v += NT) target[v] = 0;
     __syncthreads();
+    int any = 0;
     if (warm)
       for (int i = tid; i < m; i += NT) {
         const int v = warm[i];
-        if (v >= 0 && v < n + m) target[v] = 1;
+        if (v >= 0 && v < n + m) { target[v] = 1; any = 1; }
       }
+    return __syncthreads_or(any) != 0;
+  }
+
+  // dual simplex leaving row: most negative rhs below -tol (Bland: smallest basic id)
+  __device__ __forceinline__ int dual_row(bool bland, T tol) {
     __syncthreads();
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    if (gc == 0) {
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int i = row_of(k);
+        if (i >= 2 && i < R && !redund[i]) {
+          const T beta = v[k][0];
+          if (beta < -tol) {
+            const T key = bland ? T(bs[i]) : beta;
+            if (better(key, i, bv, bk)) { bv = key; bk = i; bp = i; }
+          }
+        }
+      }
+    }
+    red.argmin(bv, bk, bp);
+    return bp;
+  }
+
+  // dual simplex entering column for leaving row r (row r published through rowbuf)
+  __device__ __forceinline__ int dual_ratio(int r, bool bland, T eps_p, T& theta) {
+    const int kr = r / TR, grr = r - kr * TR;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      if (k == kr && gr == grr) {
+#pragma unroll
+        for (int l = 0; l < CPT; ++l) rowbuf[col_of(l)] = v[k][l];
+      }
+    }
+    __syncthreads();
+    col_ready = -1;
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    if (gr == 0) {
+#pragma unroll
+      for (int l = 0; l < CPT; ++l) {
+        const int j = col_of(l);
+        const T a = rowbuf[j];
+        if (!((dead >> l) & 1u) && a < -eps_p) {
+          const T d = v[0][l];
+          const T th = (d > T(0) ? d : T(0)) / -a;
+          const T key = th > T(0) ? th : a;
+          const int kk = bland ? nb[j] : j;
+          if (better(key, kk, bv, bk)) { bv = key; bk = kk; bp = j; }
+        }
+      }
+    }
+    red.argmin(bv, bk, bp);
+    theta = bv > T(0) ? bv : T(0);
+    return bp;
   }
 
   // row of largest |T[i][s]| among constraint rows whose basic variable is not a target
@@ -359,6 +431,9 @@ struct TileTab {
                                 int p1, int unb, T wfin) {
     __syncthreads();
     if (tid == 0) {
+      double am = s_am[0];
+      for (int w = 1; w < NW; ++w) am = fmax(am, s_am[w]);
+      out.amax[lp] = am;
       out.status[lp] = status;
       out.pivots[lp] = it;
       out.p1[lp] = p1;
@@ -397,8 +472,10 @@ __global__ __launch_bounds__(NW * 64) void k_tile(const LpDesc* __restrict__ des
   __shared__ int bs[Tab::RCAP], nb[Tab::CCAP];
   __shared__ uint8_t redund[Tab::RCAP], target[Tab::RCAP + Tab::CCAP];
   __shared__ RedSlot<T> slots[2 * NW];
+  __shared__ double s_am[NW];
   __shared__ int s_q;
   Tab t;
+  t.s_am = s_am;
   t.tid = threadIdx.x;
   t.gr = threadIdx.x / TC;
   t.gc = threadIdx.x - t.gr * TC;
@@ -418,7 +495,7 @@ __global__ __launch_bounds__(NW * 64) void k_tile(const LpDesc* __restrict__ des
     const int8_t* sense = d.s >= 0 ? spool + d.s : nullptr;
     t.init(pool + d.a, pool + d.b, pool + d.c, sense);
     const int32_t* warm = warm_pool ? warm_pool + d.basis : nullptr;
-    t.set_target(warm);
+    if (!t.set_target(warm)) warm = nullptr;
     int status, it, p1, unb;
     T wfin;
     run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, warm);
@@ -428,6 +505,8 @@ __global__ __launch_bounds__(NW * 64) void k_tile(const LpDesc* __restrict__ des
 
 // ---------------------------------------------------------------------------------
 // variant table: (waves per LP, TC, RPT, CPT); capacity (64*NW/TC*RPT) x (TC*CPT)
+// (the autotuner measures every variant that holds a shape; fewer waves per LP mean fewer
+// cross-wave reductions per pivot, more waves mean fewer registers per thread)
 #define BLPG_TILE_VARIANTS(X) \
   X(0, 1, 8, 4, 5)            \
   X(1, 1, 8, 7, 7)            \
@@ -435,11 +514,18 @@ __global__ __launch_bounds__(NW * 64) void k_tile(const LpDesc* __restrict__ des
   X(3, 2, 16, 10, 6)          \
   X(4, 4, 16, 7, 7)           \
   X(5, 8, 16, 6, 9)           \
-  X(6, 8, 16, 7, 13)
+  X(6, 8, 16, 7, 13)          \
+  X(7, 1, 8, 10, 11)          \
+  X(8, 1, 16, 16, 7)          \
+  X(9, 2, 16, 14, 7)          \
+  X(10, 4, 32, 14, 4)         \
+  X(11, 8, 16, 4, 7)
 
-constexpr int kTileVariants = 7;
+constexpr int kTileVariants = 12;
+constexpr int kTileDefault = 7;  // static routing uses variants 0..6
 
 int tile_variant_count(int) { return kTileVariants; }
+int tile_default_variants() { return kTileDefault; }
 
 TileVariant tile_variant(int, int v) {
   switch (v) {

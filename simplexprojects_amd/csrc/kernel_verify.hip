@@ -8,8 +8,9 @@
 //   x_S = M^-1 b'_P,   s_Q = b'_Q - A'[Q, S] x_S   (Q: rows with a basic slack),
 //   y_P = M^-T c_S,    y_Q = 0,
 // so only a k x k LU (partial pivoting, lowest row on ties) is needed, not an m x m one;
-// M sits in LDS when k <= 64 (the common case: k averages 20-60 on the mixed-size batch),
-// else in an HBM slab owned by the wavefront.
+// M sits in LDS when k <= klds, else in an HBM slab owned by the wavefront.  The solve
+// path runs a first pass with a small klds (more wavefronts per CU) that defers LPs with
+// k > klds to a list, and a second pass with klds = 64 over that list.
 // Checks, relative to (1 + |b|_inf + |A|_max |x|_inf) and (1 + |c|_inf + |A|_max |y|_inf):
 //   primal  x_S >= 0, s_Q >= 0 ('<=') / s_Q = 0 ('='), |b'_P - A'[P,S] x_S|
 //   dual    c_j - y_P^T A'[P, j] <= 0 for nonbasic structurals, -y_i <= 0 for nonbasic
@@ -30,14 +31,12 @@ namespace blpg {
 
 namespace {
 
-constexpr int KLDS = 64;  // bases with k <= 64 basic structurals factor M in LDS
-
 struct VerifyLayout {
   size_t M, S, P, pos, rhs, y, d, inb, total;
-  __host__ __device__ VerifyLayout(bool, int mcap, int ncap) {
+  __host__ __device__ VerifyLayout(int klds, int mcap, int ncap) {
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
     size_t o = 0;
-    M = o; o += al(sizeof(double) * (size_t)KLDS * KLDS);
+    M = o; o += al(sizeof(double) * (size_t)klds * klds);
     S = o; o += al(sizeof(int) * mcap);      // basic structural ids (column order of M)
     P = o; o += al(sizeof(int) * mcap);      // rows with nonbasic slack (row order of M), then Q
     pos = o; o += al(sizeof(int) * mcap);    // LU row permutation
@@ -153,10 +152,11 @@ __global__ __launch_bounds__(64) void k_verify(const LpDesc* __restrict__ desc,
                                                double tol, int fp32_source, double* ws,
                                                int64_t ws_stride, int mcap, int ncap,
                                                int32_t* queue, int32_t* fail_list,
-                                               int32_t* fail_count) {
+                                               int32_t* fail_count, int klds,
+                                               int32_t* big_list, int32_t* big_count) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_q, s_k;
-  const VerifyLayout L(true, mcap, ncap);
+  const VerifyLayout L(klds, mcap, ncap);
   double* Mlds = (double*)(smem + L.M);
   double* Mws = ws + (int64_t)blockIdx.x * ws_stride;
   int* Sl = (int*)(smem + L.S);
@@ -187,13 +187,14 @@ __global__ __launch_bounds__(64) void k_verify(const LpDesc* __restrict__ desc,
     auto sg = [&](int i) { return (sense && sense[i] == 2) ? -1.0 : 1.0; };
     auto is_eq = [&](int i) { return sense && sense[i] == 1; };
 
-    double bmax = 0, cmax = 0, am = 0;
+    double bmax = 0, cmax = 0;
     for (int i = lane; i < m; i += 64) bmax = fmax(bmax, fabs(b[i]));
     for (int j = lane; j < n; j += 64) cmax = fmax(cmax, fabs(c[j]));
-    for (int e = lane; e < m * n; e += 64) am = fmax(am, fabs(A[e]));
     bmax = wave_max(bmax);
     cmax = wave_max(cmax);
-    am = wave_max(am);
+    const double am = out.amax[lp];
+    bool defer = false;
+    int kx = -1;  // size of the FP64 basic solution in xs, once computed
 
     int ok = 2;
     double rp = -1, rd = -1, zobj = out.obj[lp];
@@ -225,6 +226,8 @@ __global__ __launch_bounds__(64) void k_verify(const LpDesc* __restrict__ desc,
       const int k = uniform(s_k);
       if (bad || k < 0) {
         ok = 0;
+      } else if (k > klds && big_list) {
+        defer = true;  // checked by the second pass with a larger LDS block
       } else {
         auto check = [&](double* Mx) {
         // ---- M = A'[P, S] and its LU ----
@@ -254,17 +257,29 @@ __global__ __launch_bounds__(64) void k_verify(const LpDesc* __restrict__ desc,
           // primal: bounds of x_S, basic slacks s_Q, residual rows P
           double pv = 0;
           for (int a = lane; a < k; a += 64) pv = fmax(pv, -xs[a]);
+          // residual rows: four independent partial sums keep several loads in flight
+          auto row_dot = [&](int i) {
+            const double* Ai = A + (int64_t)i * n;
+            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            int a = 0;
+            for (; a + 4 <= k; a += 4) {
+              s0 = fma(Ai[Sl[a]], xs[a], s0);
+              s1 = fma(Ai[Sl[a + 1]], xs[a + 1], s1);
+              s2 = fma(Ai[Sl[a + 2]], xs[a + 2], s2);
+              s3 = fma(Ai[Sl[a + 3]], xs[a + 3], s3);
+            }
+            for (; a < k; ++a) s0 = fma(Ai[Sl[a]], xs[a], s0);
+            return (s0 + s1) + (s2 + s3);
+          };
           for (int qq = k + lane; qq < m; qq += 64) {
             const int i = Pl[qq];
-            double s = sg(i) * b[i];
-            for (int a = 0; a < k; ++a) s = fma(-sg(i) * A[(int64_t)i * n + Sl[a]], xs[a], s);
+            const double s = sg(i) * (b[i] - row_dot(i));
             xm = fmax(xm, fabs(s));
             pv = fmax(pv, is_eq(i) ? fabs(s) : -s);
           }
           for (int a = lane; a < k; a += 64) {
             const int i = Pl[a];
-            double r = sg(i) * b[i];
-            for (int e2 = 0; e2 < k; ++e2) r = fma(-sg(i) * A[(int64_t)i * n + Sl[e2]], xs[e2], r);
+            const double r = sg(i) * (b[i] - row_dot(i));
             pv = fmax(pv, fabs(r));
           }
           xm = wave_max(xm);
@@ -274,22 +289,24 @@ __global__ __launch_bounds__(64) void k_verify(const LpDesc* __restrict__ desc,
           double dvv = 0;
           for (int j = lane; j < n; j += 64) {
             if (inb[j]) continue;
-            double r = c[j];
-            for (int a = 0; a < k; ++a) r = fma(-yp[a], sg(Pl[a]) * A[(int64_t)Pl[a] * n + j], r);
-            dvv = fmax(dvv, r);
+            double r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+            int a = 0;
+            for (; a + 4 <= k; a += 4) {
+              r0 = fma(yp[a] * sg(Pl[a]), A[(int64_t)Pl[a] * n + j], r0);
+              r1 = fma(yp[a + 1] * sg(Pl[a + 1]), A[(int64_t)Pl[a + 1] * n + j], r1);
+              r2 = fma(yp[a + 2] * sg(Pl[a + 2]), A[(int64_t)Pl[a + 2] * n + j], r2);
+              r3 = fma(yp[a + 3] * sg(Pl[a + 3]), A[(int64_t)Pl[a + 3] * n + j], r3);
+            }
+            for (; a < k; ++a) r0 = fma(yp[a] * sg(Pl[a]), A[(int64_t)Pl[a] * n + j], r0);
+            dvv = fmax(dvv, c[j] - ((r0 + r1) + (r2 + r3)));
           }
           for (int a = lane; a < k; a += 64)
             if (!is_eq(Pl[a])) dvv = fmax(dvv, -yp[a]);
           rd = wave_max(dvv) / (1.0 + cmax + am * ym);
-          // x (structural, FP64) and objective
-          double* x = chk.x + D.x;
-          for (int j = lane; j < n; j += 64) x[j] = 0.0;
-          __syncthreads();
+          // objective of the FP64 basic solution (x itself is written once the check passed)
+          kx = k;
           double zp = 0;
-          for (int a = lane; a < k; a += 64) {
-            x[Sl[a]] = xs[a];
-            zp = fma(c[Sl[a]], xs[a], zp);
-          }
+          for (int a = lane; a < k; a += 64) zp = fma(c[Sl[a]], xs[a], zp);
           const double zsum = wave_sum(zp);
           if (status == kStatusOptimal) {
             zobj = zsum;
@@ -330,11 +347,22 @@ __global__ __launch_bounds__(64) void k_verify(const LpDesc* __restrict__ desc,
           }
         }
         };
-        if (k <= KLDS) check(Mlds);
+        if (k <= klds) check(Mlds);
         else check(Mws);
       }
     }
     __syncthreads();
+    if (defer) {
+      if (lane == 0) big_list[atomicAdd(big_count, 1)] = lp;
+      continue;
+    }
+    // a verified result reports the FP64 basic solution: x_S from the LU solve, 0 elsewhere
+    if (ok == 1 && kx >= 0) {
+      double* x = out.x + D.x;
+      for (int j = lane; j < n; j += 64) x[j] = 0.0;
+      __syncthreads();
+      for (int a = lane; a < kx; a += 64) x[Sl[a]] = xs[a];
+    }
     if (lane == 0) {
       chk.ok[lp] = ok;
       chk.obj[lp] = zobj;
@@ -355,25 +383,25 @@ __global__ __launch_bounds__(64) void k_verify(const LpDesc* __restrict__ desc,
   }
 }
 
-size_t verify_lds_bytes(int mcap, int ncap, bool lds) { return VerifyLayout(lds, mcap, ncap).total; }
+size_t verify_lds_bytes(int mcap, int ncap, int klds) { return VerifyLayout(klds, mcap, ncap).total; }
 
-int verify_max_blocks_per_cu(int mcap, int ncap, bool lds) {
+int verify_max_blocks_per_cu(int mcap, int ncap, int klds) {
   int nb = 0;
   const void* fn = reinterpret_cast<const void*>(&k_verify);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, verify_lds_bytes(mcap, ncap, lds)) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, verify_lds_bytes(mcap, ncap, klds)) !=
       hipSuccess)
     return 0;
   return nb;
 }
 
 hipError_t launch_verify(const VerifyLaunch& V, hipStream_t st) {
-  const size_t bytes = verify_lds_bytes(V.mcap, V.ncap, true);
+  const size_t bytes = verify_lds_bytes(V.mcap, V.ncap, V.klds);
   const void* fn = reinterpret_cast<const void*>(&k_verify);
   void* args[] = {(void*)&V.desc, (void*)&V.pool, (void*)&V.spool, (void*)&V.idx,
                   (void*)&V.count, (void*)&V.count_dev, (void*)&V.out, (void*)&V.chk,
                   (void*)&V.tol, (void*)&V.fp32_source, (void*)&V.ws, (void*)&V.ws_stride,
                   (void*)&V.mcap, (void*)&V.ncap, (void*)&V.queue, (void*)&V.fail_list,
-                  (void*)&V.fail_count};
+                  (void*)&V.fail_count, (void*)&V.klds, (void*)&V.big_list, (void*)&V.big_count};
   return hipLaunchKernel(fn, dim3(V.grid), dim3(64), args, bytes, st);
 }
 

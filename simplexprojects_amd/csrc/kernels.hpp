@@ -57,6 +57,7 @@ struct TileVariant {
   int ccap() const { return (nw * 64 / tr) * cpt; }
 };
 int tile_variant_count(int precision);
+int tile_default_variants();  // variants 0..k-1 form the static routing order
 TileVariant tile_variant(int precision, int v);
 int tile_max_blocks_per_cu(int precision, int v);
 hipError_t launch_tile(const TileLaunch& L, hipStream_t st);
@@ -80,10 +81,13 @@ struct VerifyLaunch {
   int32_t* queue;
   int32_t* fail_list;  // LP ids that failed the check (appended)
   int32_t* fail_count;
+  int klds = 64;       // bases with k <= klds basic structurals factor M in LDS
+  int32_t* big_list = nullptr;   // if set, LPs with k > klds are appended here unchecked
+  int32_t* big_count = nullptr;
 };
 hipError_t launch_verify(const VerifyLaunch& L, hipStream_t st);
-int verify_max_blocks_per_cu(int mcap, int ncap, bool lds);
-size_t verify_lds_bytes(int mcap, int ncap, bool lds);
+int verify_max_blocks_per_cu(int mcap, int ncap, int klds);
+size_t verify_lds_bytes(int mcap, int ncap, int klds);
 
 // small helpers (kernel_util.hip)
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);

@@ -191,12 +191,19 @@ __device__ __forceinline__ T init_entry(int i, int j, int m, int n, const double
 // machine so that every backend's pivot / pricing / ratio code is inlined exactly once.
 // Statement for statement the same decisions, in the same order, as the oracle's
 // solve_impl (oracle/simplex_core.inc), which is what makes the results bit-identical.
-enum : int { kStCrash = 0, kStEq = 1, kStAux = 2, kStP1 = 3, kStP1End = 4, kStP2 = 5 };
+enum : int {
+  kStCrash = 0, kStEq = 1, kStAux = 2, kStP1 = 3, kStP1End = 4, kStP2 = 5, kStDualChk = 6,
+  kStDual = 7
+};
 
 // `warm` (optional, m entries): a target basis (variable ids, -1 = none).  Before phase I
 // its structurals are pivoted in one by one (crash), each on the row of largest |T[i][s]|
 // among rows whose basic variable is not itself a target; a structural with no eligible
-// nonzero is skipped (its row keeps the slack).  Phase I / II then repair and finish.
+// nonzero is skipped (its row keeps the slack).  Repair: when the crashed basis is dual
+// feasible (no live column with T[0][j] < -eps_d), dual simplex pivots (leaving row: most
+// negative rhs below -tol; entering column: smallest max(T[0][j],0) / -T[r][j] over
+// T[r][j] < -eps_p, dual-degenerate columns by the largest |T[r][j]|) restore primal
+// feasibility; then the auxiliary phase I (if still needed) and phase II finish.
 template <typename T, class Tab>
 __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* sense,
                                             const Opt<T>& o, int& status, int& it, int& p1,
@@ -216,7 +223,7 @@ __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* 
         ++ci;
         if (w >= 0 && w < n) v = w;
       }
-      if (v < 0) { stage = sense ? kStEq : kStAux; continue; }
+      if (v < 0) { stage = sense ? kStEq : kStDualChk; continue; }
       const int s = t.find_nonbasic(v);
       if (s < 0) continue;
       if (it >= maxit) { status = kStatusIterLimit; break; }
@@ -227,7 +234,7 @@ __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* 
       kill_after = lv >= n && lv < n + m && sense && sense[lv - n] == 1;
     } else if (stage == kStEq) {
       while (ei < m && sense[ei] != 1) ++ei;
-      if (ei >= m) { stage = kStAux; continue; }
+      if (ei >= m) { stage = warm ? kStDualChk : kStAux; continue; }
       const int r = 2 + ei;
       ++ei;
       if (t.bs_of(r) != n + r - 2) continue;  // its slack already left (crash)
@@ -240,6 +247,20 @@ __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* 
         break;
       }
       pr = r; ps = j; kill_after = true; count_p1 = true;
+    } else if (stage == kStDualChk) {
+      deg = 0;
+      stage = t.pricing(0, false, o.eps_d) < 0 ? kStDual : kStAux;
+      continue;
+    } else if (stage == kStDual) {
+      const bool bland = deg >= o.bland_after;
+      const int r = t.dual_row(bland, o.tol);
+      if (r < 0) { stage = kStAux; continue; }
+      if (it >= maxit) { status = kStatusIterLimit; break; }
+      T th;
+      const int s = t.dual_ratio(r, bland, o.eps_p, th);
+      if (s < 0) { stage = kStAux; continue; }
+      pr = r; ps = s; count_p1 = true;
+      deg = (th == T(0)) ? deg + 1 : 0;
     } else if (stage == kStAux) {
       const int rmin = t.min_rhs_row();
       deg = 0;

@@ -6,13 +6,14 @@
 //   LDS    the streamed kernel with the tableau in LDS (when it fits the CU's LDS),
 //   STREAM the streamed kernel with the tableau in an HBM slab (any size).
 // All three perform identical arithmetic, so the choice changes speed, never results.
-// A measured preference table (set_preference, filled by the benchmark's autotune
-// pass and optionally loaded from a cache file) overrides the static order.
+// A measured preference table (set_preference, filled by blpg_autotune or imported as
+// text) overrides the static order.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <map>
+#include <string>
 #include <utility>
 
 #include "../../include/blpg/blpg.h"
@@ -42,11 +43,43 @@ class Tuner {
     return 256;
   }
 
-  // preference key: (precision, m, n) -> (kernel, variant or nt)
-  void set_preference(int precision, int m, int n, int kernel, int param) {
+  // preference key: (precision, m, n) -> (kernel, variant or nt), with the measured time
+  void set_preference(int precision, int m, int n, int kernel, int param, double ms = -1.0) {
     pref_[key(precision, m, n)] = {kernel, param};
+    ms_[key(precision, m, n)] = ms;
   }
-  void clear_preferences() { pref_.clear(); }
+  void clear_preferences() { pref_.clear(); ms_.clear(); }
+  // "precision m n kernel param ms" lines
+  std::string export_text() const {
+    std::string out;
+    char line[128];
+    for (const auto& kv : pref_) {
+      const long k = kv.first;
+      const int p = (int)(k >> 40), m = (int)((k >> 20) & 0xFFFFF), n = (int)(k & 0xFFFFF);
+      const auto it = ms_.find(k);
+      snprintf(line, sizeof line, "%d %d %d %d %d %.6f\n", p, m, n, kv.second.first,
+               kv.second.second, it == ms_.end() ? -1.0 : it->second);
+      out += line;
+    }
+    return out;
+  }
+  bool import_text(const char* text) {
+    const char* p = text;
+    while (p && *p) {
+      int pr, m, n, kern, param, used = 0;
+      double ms = -1;
+      if (sscanf(p, "%d %d %d %d %d %lf%n", &pr, &m, &n, &kern, &param, &ms, &used) < 6) {
+        while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t') ++p;
+        if (!*p) break;
+        return false;
+      }
+      if (pr < 0 || pr > 1 || m < 1 || n < 1 || m > 4096 || n > 4096 || kern < 1 || kern > 3)
+        return false;
+      set_preference(pr, m, n, kern, param, ms);
+      p += used;
+    }
+    return true;
+  }
 
   Route route(int m, int n, int precision, int forced) const {
     Route r{};
@@ -89,11 +122,12 @@ class Tuner {
     return t.rcap() >= m + 2 && t.ccap() >= n + 2;
   }
 
-  // smallest-capacity tile variant that holds the LP
+  // smallest-capacity tile variant of the default set that holds the LP (the remaining
+  // variants are chosen only by measurement)
   static int best_tile(int m, int n, int precision) {
     int best = -1;
     long best_cap = 0;
-    for (int v = 0; v < tile_variant_count(precision); ++v) {
+    for (int v = 0; v < tile_default_variants(); ++v) {
       if (!fits_tile(v, m, n, precision)) continue;
       const TileVariant t = tile_variant(precision, v);
       const long cap = (long)t.rcap() * t.ccap();
@@ -106,6 +140,7 @@ class Tuner {
   static long key(int p, int m, int n) { return ((long)p << 40) | ((long)m << 20) | n; }
   size_t lds_ = 65536;
   std::map<long, std::pair<int, int>> pref_;
+  std::map<long, double> ms_;
 };
 
 }  // namespace blpg

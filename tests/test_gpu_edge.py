@@ -123,3 +123,23 @@ def test_cpp_drop_in_program():
     r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                        timeout=120)
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fetch_matches_download(precision):
+    lps = sa.gen_config(2, 2, 56, 97, 64, shape_idx=1, batch_count=1024)
+    S = sa.Solver(precision=precision)
+    b = S.upload(lps)
+    b.solve()
+    d = b.download()
+    f = b.fetch()
+    for name in ("status", "pivots", "phase1_pivots", "fallback", "objective", "kkt_primal",
+                 "kkt_dual"):
+        assert np.array_equal(getattr(d, name), getattr(f, name)), name
+    for k in range(len(lps)):
+        assert np.array_equal(d.x[k], f.x[k]) and np.array_equal(d.basis[k], f.basis[k])
+        if d.status[k] == sa.OPTIMAL:  # verified x is the FP64 basic solution
+            assert abs(float(lps[k].c @ f.x[k]) - f.objective[k]) <= 1e-9 * max(1, abs(f.objective[k]))
+            assert (lps[k].A @ f.x[k] <= lps[k].b + 1e-9).all() and (f.x[k] >= 0).all()
+    b.free()
+    S.close()

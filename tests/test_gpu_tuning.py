@@ -1,0 +1,61 @@
+"""GPU tests of routing and measured autotuning: every kernel instance (each register-tile
+variant, LDS-resident and HBM-streamed tableaux at each workgroup size) replays the oracle
+bit for bit, so the autotuner's choice changes speed only; the tuning table round-trips
+through its text form; invalid pins are rejected."""
+import numpy as np
+import pytest
+
+import oracle
+import simplexprojects_amd as sa
+
+pytestmark = pytest.mark.gpu
+
+INSTANCES = [("tile", v) for v in range(12)] + [(k, nt) for k in ("lds", "stream")
+                                                 for nt in (64, 128, 256, 512)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("kernel,param", INSTANCES)
+def test_every_instance_is_bit_exact(kernel, param, precision):
+    lps = sa.gen_config(2, 2, 30, 38, 12, shape_idx=0, batch_count=1024)
+    S = sa.Solver(precision=precision, verify=False)
+    S.tune_set(30, 38, kernel, param)
+    b = S.upload(lps)
+    b.solve()
+    assert b.info()["bins"][0]["kernel"] == sa._KERNELS[kernel]
+    r = b.fetch()
+    for k, lp in enumerate(lps):
+        if r.fallback[k]:  # FP32 result failed the FP64 check: re-solved in FP64
+            assert r.status[k] == oracle.solve(lp.A, lp.b, lp.c)["status"]
+            continue
+        o = oracle.solve(lp.A, lp.b, lp.c, precision=precision)
+        assert r.status[k] == o["status"] and r.pivots[k] == o["pivots"], (k, kernel, param)
+        assert np.array_equal(r.basis[k], o["basis"])
+        if precision == "fp64":  # (FP32 results report the FP64 re-solve's objective)
+            assert r.objective[k] == o["objective"]
+    b.free()
+    S.close()
+
+
+def test_autotune_table_and_results():
+    lps = (sa.gen_config(2, 2, 50, 48, 256, shape_idx=0, batch_count=1024) +
+           sa.gen_config(2, 2, 74, 83, 256, shape_idx=2, batch_count=1024))
+    S = sa.Solver(precision="fp64")
+    base = S.solve(lps)
+    table = S.autotune(lps, reps=1)
+    shapes = {(m, n) for (_, m, n, _, _, _) in table}
+    assert shapes == {(50, 48), (74, 83)}
+    assert all(ms > 0 for *_, ms in table)
+    tuned = S.solve(lps)
+    assert np.array_equal(base.pivots, tuned.pivots)
+    assert np.array_equal(base.objective, tuned.objective)
+    text = S.tune_export()
+    S2 = sa.Solver(precision="fp64")
+    S2.tune_import(text)
+    assert S2.tune_export() == text
+    with pytest.raises(sa.BlpgError):
+        S2.tune_import("0 50 48 9 0 1.0\n")
+    with pytest.raises(sa.BlpgError):
+        S2.tune_set(200, 200, "tile", 0)  # variant 0 holds 30x38 at most
+    S.close()
+    S2.close()

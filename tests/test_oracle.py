@@ -225,3 +225,22 @@ def test_generator_statistics_and_subsets():
         assert np.array_equal(sub[k].A, lps[500 + k].A) and np.array_equal(sub[k].c, lps[500 + k].c)
     l3 = sa.gen_config(3, 3, 174, 142, 200, shape_idx=0, batch_count=8192)
     assert all(int((lp.sense == 1).sum()) in (52, 51) for lp in l3 if not lp.flags & 2)
+
+
+def test_dual_repair_after_rhs_change():
+    # the optimal basis of one instance, crashed into an instance with a changed b, stays
+    # dual feasible; dual simplex pivots restore primal feasibility (HiGHS pins the answer)
+    rng = np.random.default_rng(5)
+    lps = sa.gen_config(4, 4, 105, 103, 12, shape_idx=1, batch_count=1024)
+    dual_used = 0
+    for lp in lps:
+        r0 = oracle.solve(lp.A, lp.b, lp.c)
+        b2 = lp.b * (1 + 0.3 * rng.standard_normal(lp.m))
+        b2 = np.maximum(b2, 0.1)
+        w = oracle.solve(lp.A, b2, lp.c, warm=r0["basis"])
+        st, obj = highs(lp.A, b2, lp.c)
+        assert w["status"] == st == oracle.OPTIMAL
+        assert rel_err(w["objective"], obj) <= 1e-9
+        crash = int((r0["basis"] < lp.n).sum())
+        dual_used += w["phase1_pivots"] > crash
+    assert dual_used >= 3
