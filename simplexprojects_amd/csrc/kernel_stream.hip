@@ -35,6 +35,7 @@ struct StreamTab {
   Reducer<T, NW> red;
 
   __device__ __forceinline__ T& at(int i, int j) { return tab[(int64_t)i * ld + j]; }
+  __device__ __forceinline__ void loop_fence() {}
 
   __device__ __forceinline__ void init(const double* A, const double* b, const double* c, const int8_t* sense) {
     const int tid = threadIdx.x;

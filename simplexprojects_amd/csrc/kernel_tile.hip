@@ -6,10 +6,15 @@
 // `v[RPT][CPT]`.  A CU's 512 KB of VGPRs is 3.2x its LDS, so tableaux up to ~207x205 FP64
 // stay on-chip, and the rank-1 update is RPT*CPT register FMAs per thread with no memory
 // traffic at all.  Per pivot LDS carries only
-//   colbuf  the pivot column (written by the TR owners of column s, 16 B per row)
-//   rhsbuf  the right-hand side column (for the ratio test)
+//   colbuf  the pivot column (written by the TR owners of column s)
+//   rhsbuf  the right-hand side column, kept current by its owners after every update
 //   rowbuf  the scaled pivot row (written by the TC owners of row r)
-// and the (value, index) reduction partials.  Indices of the pivot row / column are
+// Division of labour per pivot: every (value, index) reduction -- pricing over row 0
+// (whose owners are lanes 0..TC-1 of wavefront 0), ratio over the published column, and
+// the phase-I / crash / dual selections -- runs on wavefront 0 alone and its result is
+// broadcast through LDS; the other wavefronts wait at the barrier without issuing
+// instructions, so the SIMDs they share with other workgroups stay free.  Only the
+// rank-1 update uses every wavefront.  Indices of the pivot row / column are
 // wave-uniform, so their extraction from the register block is a scalar branch over the
 // unrolled k / l slots, not a dynamic register index.
 //
@@ -23,21 +28,28 @@
 
 namespace blpg {
 
+// wavefront 0's decision, broadcast to the workgroup
+template <typename T>
+struct Decision {
+  T v;
+  int k, p;
+};
+
 template <typename T, int NW, int TC, int RPT, int CPT>
 struct TileTab {
   static constexpr int NT = NW * 64;
   static constexpr int TR = NT / TC;
   static constexpr int RCAP = TR * RPT;
   static constexpr int CCAP = TC * CPT;
-  static_assert(TC <= 64 && 64 % TC == 0, "a row group must sit inside one wavefront");
-  static_assert(TR >= 2, "rows 0 and 1 must be in slot k = 0");
+  static_assert(TC <= 32 && 64 % TC == 0, "rows 0 and 1 must sit in wavefront 0");
   static_assert(CPT <= 32, "dead-column mask is 32 bits");
 
   T v[RPT][CPT];
   uint32_t dead;  // bit l: column gc + TC*l cannot enter
-  int gr, gc, tid;
+  int gr, gc, tid, lane, wv;
   int R, C, m, n;
   int col_ready;  // column whose entries are in colbuf (-1: none)
+  int phase;      // broadcast slot
   T* colbuf;
   T* rhsbuf;
   T* rowbuf;
@@ -47,10 +59,34 @@ struct TileTab {
   uint8_t* redund;
   uint8_t* target;  // target[var] = 1: var belongs to the warm-start basis
   double* s_am;     // per-wave max |A_ij| (NW entries)
-  Reducer<T, NW> red;
+  Decision<T>* slots;  // 2 broadcast slots
+
+  // Called once per state-machine step: the per-thread indices become opaque values, so
+  // the compiler recomputes index arithmetic where it is used instead of hoisting every
+  // stage's invariants out of the loop (which holds registers for the whole kernel and
+  // costs a wave per SIMD of occupancy).
+  __device__ __forceinline__ void loop_fence() { asm volatile("" : "+v"(gr), "+v"(gc), "+v"(tid), "+v"(lane)); }
 
   __device__ __forceinline__ int row_of(int k) const { return gr + TR * k; }
   __device__ __forceinline__ int col_of(int l) const { return gc + TC * l; }
+
+  // wavefront 0's (wave-uniform) reduction result to every wavefront
+  __device__ __forceinline__ void bcast(T& bv, int& bk, int& bp) {
+    Decision<T>* sl = slots + phase;
+    if (tid == 0) { sl->v = bv; sl->k = bk; sl->p = bp; }
+    __syncthreads();
+    bv = uniform(sl->v);
+    bk = uniform(sl->k);
+    bp = uniform(sl->p);
+    phase ^= 1;
+  }
+
+  __device__ __forceinline__ void publish_rhs() {
+    if (gc == 0) {
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) rhsbuf[row_of(k)] = v[k][0];
+    }
+  }
 
   __device__ __forceinline__ void init(const double* A, const double* b, const double* c, const int8_t* sense) {
 #pragma unroll
@@ -81,103 +117,25 @@ struct TileTab {
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) am = fmax(am, __shfl_xor(am, off));
-    if ((tid & 63) == 0) s_am[tid >> 6] = am;
+    if (lane == 0) s_am[tid >> 6] = am;
     for (int i = tid; i < RCAP; i += NT) { bs[i] = (i >= 2 && i < R) ? n + i - 2 : -1; redund[i] = 0; }
     for (int j = tid; j < CCAP; j += NT) nb[j] = (j == 0 || j >= C) ? -1 : (j <= n ? j - 1 : n + m);
+    publish_rhs();
     col_ready = -1;
     __syncthreads();
   }
 
   // marks the warm-start targets; false when the list holds no valid variable id
   __device__ __forceinline__ bool set_target(const int32_t* warm) {
-    for (int v = tid; v < RCAP + CCAP; v += NT) target[v] = 0;
+    for (int x = tid; x < RCAP + CCAP; x += NT) target[x] = 0;
     __syncthreads();
     int any = 0;
     if (warm)
       for (int i = tid; i < m; i += NT) {
-        const int v = warm[i];
-        if (v >= 0 && v < n + m) { target[v] = 1; any = 1; }
+        const int x = warm[i];
+        if (x >= 0 && x < n + m) { target[x] = 1; any = 1; }
       }
     return __syncthreads_or(any) != 0;
-  }
-
-  // dual simplex leaving row: most negative rhs below -tol (Bland: smallest basic id)
-  __device__ __forceinline__ int dual_row(bool bland, T tol) {
-    __syncthreads();
-    T bv = inf_of<T>();
-    int bk = INT_MAX, bp = -1;
-    if (gc == 0) {
-#pragma unroll
-      for (int k = 0; k < RPT; ++k) {
-        const int i = row_of(k);
-        if (i >= 2 && i < R && !redund[i]) {
-          const T beta = v[k][0];
-          if (beta < -tol) {
-            const T key = bland ? T(bs[i]) : beta;
-            if (better(key, i, bv, bk)) { bv = key; bk = i; bp = i; }
-          }
-        }
-      }
-    }
-    red.argmin(bv, bk, bp);
-    return bp;
-  }
-
-  // dual simplex entering column for leaving row r (row r published through rowbuf)
-  __device__ __forceinline__ int dual_ratio(int r, bool bland, T eps_p, T& theta) {
-    const int kr = r / TR, grr = r - kr * TR;
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      if (k == kr && gr == grr) {
-#pragma unroll
-        for (int l = 0; l < CPT; ++l) rowbuf[col_of(l)] = v[k][l];
-      }
-    }
-    __syncthreads();
-    col_ready = -1;
-    T bv = inf_of<T>();
-    int bk = INT_MAX, bp = -1;
-    if (gr == 0) {
-#pragma unroll
-      for (int l = 0; l < CPT; ++l) {
-        const int j = col_of(l);
-        const T a = rowbuf[j];
-        if (!((dead >> l) & 1u) && a < -eps_p) {
-          const T d = v[0][l];
-          const T th = (d > T(0) ? d : T(0)) / -a;
-          const T key = th > T(0) ? th : a;
-          const int kk = bland ? nb[j] : j;
-          if (better(key, kk, bv, bk)) { bv = key; bk = kk; bp = j; }
-        }
-      }
-    }
-    red.argmin(bv, bk, bp);
-    theta = bv > T(0) ? bv : T(0);
-    return bp;
-  }
-
-  // row of largest |T[i][s]| among constraint rows whose basic variable is not a target
-  __device__ __forceinline__ int absmax_row(int s, T eps_p) {
-    __syncthreads();
-    publish_col(s);
-    __syncthreads();
-    col_ready = s;
-    T bv = inf_of<T>();
-    int bk = INT_MAX, bp = -1;
-    for (int i = 2 + tid; i < R; i += NT) {
-      const int v = bs[i];
-      if (redund[i] || v < 0 || target[v]) continue;
-      const T key = -tabs(colbuf[i]);
-      if (better(key, i, bv, bk)) { bv = key; bk = i; bp = i; }
-    }
-    red.argmin(bv, bk, bp);
-    if (bp < 0 || !(-bv > eps_p)) return -1;
-    return bp;
-  }
-
-  __device__ __forceinline__ int bs_of(int i) {
-    __syncthreads();
-    return uniform(bs[i]);
   }
 
   // the TR owners of column s store it in colbuf
@@ -194,55 +152,90 @@ struct TileTab {
     }
   }
 
+  // the TC owners of row r store it (unscaled) in rowbuf
+  __device__ __forceinline__ void publish_row(int r) {
+    const int kr = r / TR, grr = r - kr * TR;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      if (k == kr && gr == grr) {
+#pragma unroll
+        for (int l = 0; l < CPT; ++l) rowbuf[col_of(l)] = v[k][l];
+      }
+    }
+  }
+
+  // row of largest |T[i][s]| among constraint rows whose basic variable is not a target
+  __device__ __forceinline__ int absmax_row(int s, T eps_p) {
+    __syncthreads();
+    publish_col(s);
+    __syncthreads();
+    col_ready = s;
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    if (wv == 0) {
+      ArgMin<T> am;
+      for (int i = 2 + lane; i < R; i += 64) {
+        const int x = bs[i];
+        const bool ok = !redund[i] && x >= 0 && !target[x < 0 ? 0 : x];
+        am.add(ok, -tabs(colbuf[i]), i, i);
+      }
+      wave_finish(am, bv, bk, bp);
+    }
+    bcast(bv, bk, bp);
+    if (bp < 0 || !(-bv > eps_p)) return -1;
+    return bp;
+  }
+
+  __device__ __forceinline__ int bs_of(int i) {
+    __syncthreads();
+    return uniform(bs[i]);
+  }
+
+  // Dantzig pricing on objective row `orow` (lanes orow*TC .. orow*TC+TC-1 of wavefront 0)
   __device__ __forceinline__ int pricing(int orow, bool bland, T eps_d) {
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
-    if (gr == orow) {
-      if (bland) {
+    if (wv == 0) {
+      ArgMin<T> am;
+      if (gr == orow) {
+        if (bland) {
 #pragma unroll
-        for (int l = 0; l < CPT; ++l) {
-          const int j = col_of(l);
-          const T d = v[0][l];
-          if (!((dead >> l) & 1u) && d < -eps_d) {
-            const T key = T(nb[j]);
-            if (better(key, j, bv, bk)) { bv = key; bk = j; bp = j; }
+          for (int l = 0; l < CPT; ++l) {
+            const int j = col_of(l);
+            am.add(!((dead >> l) & 1u) && v[0][l] < -eps_d, T(nb[j]), j, j);
+          }
+        } else {
+#pragma unroll
+          for (int l = 0; l < CPT; ++l) {
+            const int j = col_of(l);
+            am.add(!((dead >> l) & 1u) && v[0][l] < -eps_d, v[0][l], j, j);
           }
         }
-      } else {
-#pragma unroll
-        for (int l = 0; l < CPT; ++l) {
-          const int j = col_of(l);
-          const T d = v[0][l];
-          if (!((dead >> l) & 1u) && d < -eps_d && better(d, j, bv, bk)) { bv = d; bk = j; bp = j; }
-        }
       }
+      wave_finish(am, bv, bk, bp);
     }
-    red.argmin(bv, bk, bp);
+    bcast(bv, bk, bp);
     return bp;
   }
 
   __device__ __forceinline__ int ratio(int s, bool bland, T eps_p, T& theta) {
     publish_col(s);
-    if (gc == 0) {
-#pragma unroll
-      for (int k = 0; k < RPT; ++k) rhsbuf[row_of(k)] = v[k][0];
-    }
     __syncthreads();
     col_ready = s;
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
-    for (int i = tid; i < R; i += NT) {
-      if (i < 2) continue;
-      const T a = colbuf[i];
-      if (a > eps_p) {
+    if (wv == 0) {
+      ArgMin<T> am;
+      for (int i = 2 + lane; i < R; i += 64) {
+        const T a = colbuf[i];
         const T beta = rhsbuf[i];
-        const T th = (beta > T(0) ? beta : T(0)) / a;
-        const T v = th > T(0) ? th : -a;  // degenerate rows: largest pivot first
-        const int key = bland ? bs[i] : i;
-        if (better(v, key, bv, bk)) { bv = v; bk = key; bp = i; }
+        const T th = (beta > T(0) ? beta : T(0)) / (a > eps_p ? a : T(1));
+        const T key = th > T(0) ? th : -a;  // degenerate rows: largest pivot first
+        am.add(a > eps_p, key, bland ? bs[i] : i, i);
       }
+      wave_finish(am, bv, bk, bp);
     }
-    red.argmin(bv, bk, bp);
+    bcast(bv, bk, bp);
     theta = bv > T(0) ? bv : T(0);
     return bp;
   }
@@ -253,14 +246,13 @@ struct TileTab {
       publish_col(s);
       __syncthreads();
     }
-    const T p = uniform(colbuf[r]);
-    const T rp = T(1) / p;
     const int kr = r / TR, grr = r - kr * TR;
     const int ls = s / TC, gcs = s - ls * TC;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
       if (k == kr) {
         if (gr == grr) {
+          const T rp = T(1) / colbuf[r];
 #pragma unroll
           for (int l = 0; l < CPT; ++l) {
             const int j = col_of(l);
@@ -296,40 +288,84 @@ struct TileTab {
         for (int l = 0; l < CPT; ++l) v[k][l] = own ? prow[l] : v[k][l];
       }
     }
+    publish_rhs();
     col_ready = -1;
   }
 
+  // column of largest |T[r][j]| among live columns; -1 if all <= eps_p
   __device__ __forceinline__ int absmax_col(int r, T eps_p) {
+    __syncthreads();
+    publish_row(r);
+    __syncthreads();
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
-    const int kr = r / TR, grr = r - kr * TR;
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      if (k == kr && gr == grr) {
+    if (wv == 0) {
+      ArgMin<T> am;
+      if (gr == 0) {
 #pragma unroll
         for (int l = 0; l < CPT; ++l) {
           const int j = col_of(l);
-          const T key = -tabs(v[k][l]);
-          if (!((dead >> l) & 1u) && better(key, j, bv, bk)) { bv = key; bk = j; bp = j; }
+          am.add(!((dead >> l) & 1u), -tabs(rowbuf[j]), j, j);
         }
       }
+      wave_finish(am, bv, bk, bp);
     }
-    red.argmin(bv, bk, bp);
+    bcast(bv, bk, bp);
     if (bp < 0 || !(-bv > eps_p)) return -1;
     return bp;
   }
 
+  // dual simplex leaving row: most negative rhs below -tol (Bland: smallest basic id)
+  __device__ __forceinline__ int dual_row(bool bland, T tol) {
+    __syncthreads();
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    if (wv == 0) {
+      ArgMin<T> am;
+      for (int i = 2 + lane; i < R; i += 64) {
+        const T beta = rhsbuf[i];
+        am.add(!redund[i] && beta < -tol, bland ? T(bs[i]) : beta, i, i);
+      }
+      wave_finish(am, bv, bk, bp);
+    }
+    bcast(bv, bk, bp);
+    return bp;
+  }
+
+  // dual simplex entering column for leaving row r (row r published through rowbuf; the
+  // objective row is read from the registers of its owners)
+  __device__ __forceinline__ int dual_ratio(int r, bool bland, T eps_p, T& theta) {
+    publish_row(r);
+    __syncthreads();
+    col_ready = -1;
+    T bv = inf_of<T>();
+    int bk = INT_MAX, bp = -1;
+    if (wv == 0) {
+      ArgMin<T> am;
+      if (gr == 0) {
+#pragma unroll
+        for (int l = 0; l < CPT; ++l) {
+          const int j = col_of(l);
+          const T a = rowbuf[j];
+          const T d = v[0][l];
+          const bool ok = !((dead >> l) & 1u) && a < -eps_p;
+          const T th = (d > T(0) ? d : T(0)) / (ok ? -a : T(1));
+          const T key = th > T(0) ? th : a;
+          am.add(ok, key, bland ? nb[j] : j, j);
+        }
+      }
+      wave_finish(am, bv, bk, bp);
+    }
+    bcast(bv, bk, bp);
+    theta = bv > T(0) ? bv : T(0);
+    return bp;
+  }
+
+  // A dead column never enters again, and its entries influence no other entry (the
+  // update of (i, j) reads only column s and row r), so killing it only sets its bit.
   __device__ __forceinline__ void kill_col(int j) {
     const int lj = j / TC, gcj = j - lj * TC;
-#pragma unroll
-    for (int l = 0; l < CPT; ++l) {
-      if (l == lj) {
-        const bool own = gc == gcj;
-#pragma unroll
-        for (int k = 0; k < RPT; ++k) v[k][l] = own ? T(0) : v[k][l];
-        if (own) dead |= 1u << l;
-      }
-    }
+    if (gc == gcj) dead |= 1u << lj;
   }
 
   __device__ __forceinline__ T get(int i, int j) {
@@ -353,6 +389,7 @@ struct TileTab {
 #pragma unroll
       for (int l = 0; l < CPT; ++l)
         if (k == ki && l == lj) v[k][l] = (gr == gri && gc == gcj) ? val : v[k][l];
+    if (j == 0 && gr == gri && gc == 0) rhsbuf[i] = val;
   }
 
   __device__ __forceinline__ void mark_redund(int r) {
@@ -366,17 +403,12 @@ struct TileTab {
     __syncthreads();
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
-    if (gc == 0) {
-#pragma unroll
-      for (int k = 0; k < RPT; ++k) {
-        const int i = row_of(k);
-        if (i >= 2 && i < R && !redund[i]) {
-          const T val = v[k][0];
-          if (better(val, i, bv, bk)) { bv = val; bk = i; bp = i; }
-        }
-      }
+    if (wv == 0) {
+      ArgMin<T> am;
+      for (int i = 2 + lane; i < R; i += 64) am.add(!redund[i], rhsbuf[i], i, i);
+      wave_finish(am, bv, bk, bp);
     }
-    red.argmin(bv, bk, bp);
+    bcast(bv, bk, bp);
     return bp;
   }
 
@@ -406,9 +438,12 @@ struct TileTab {
     __syncthreads();
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
-    for (int i = 2 + tid; i < R; i += NT)
-      if (bs[i] == var) { bv = T(0); bk = i; bp = i; }
-    red.argmin(bv, bk, bp);
+    if (wv == 0) {
+      ArgMin<T> am;
+      for (int i = 2 + lane; i < R; i += 64) am.add(bs[i] == var, T(0), i, i);
+      wave_finish(am, bv, bk, bp);
+    }
+    bcast(bv, bk, bp);
     return bp;
   }
 
@@ -416,9 +451,12 @@ struct TileTab {
     __syncthreads();
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
-    for (int j = 1 + tid; j < C; j += NT)
-      if (nb[j] == var) { bv = T(0); bk = j; bp = j; }
-    red.argmin(bv, bk, bp);
+    if (wv == 0) {
+      ArgMin<T> am;
+      for (int j = 1 + lane; j < C; j += 64) am.add(nb[j] == var, T(0), j, j);
+      wave_finish(am, bv, bk, bp);
+    }
+    bcast(bv, bk, bp);
     return bp;
   }
 
@@ -427,6 +465,7 @@ struct TileTab {
     return uniform(nb[j]);
   }
 
+  // column 0 is mirrored in rhsbuf (objective T[0][0], basic values T[i][0])
   __device__ __forceinline__ void write_outputs(int lp, const LpDesc& d, const Out& out, int status, int it,
                                 int p1, int unb, T wfin) {
     __syncthreads();
@@ -438,7 +477,7 @@ struct TileTab {
       out.pivots[lp] = it;
       out.p1[lp] = p1;
       out.unb[lp] = unb;
-      out.obj[lp] = (double)v[0][0];
+      out.obj[lp] = (double)rhsbuf[0];
       out.wfin[lp] = (double)wfin;
     }
     double* x = out.x + d.x;
@@ -446,21 +485,16 @@ struct TileTab {
     for (int j = tid; j < n; j += NT) x[j] = 0.0;
     for (int i = tid; i < m; i += NT) basis[i] = bs[2 + i];
     __syncthreads();
-    if (gc == 0) {
-#pragma unroll
-      for (int k = 0; k < RPT; ++k) {
-        const int i = row_of(k);
-        if (i >= 2 && i < R) {
-          const int var = bs[i];
-          if (var >= 0 && var < n) x[var] = (double)v[k][0];
-        }
-      }
+    for (int i = 2 + tid; i < R; i += NT) {
+      const int var = bs[i];
+      if (var >= 0 && var < n) x[var] = (double)rhsbuf[i];
     }
   }
 };
 
-template <typename T, int NW, int TC, int RPT, int CPT>
-__global__ __launch_bounds__(NW * 64) void k_tile(const LpDesc* __restrict__ desc,
+// WPE: minimum waves per SIMD the register allocator must allow (1: no constraint)
+template <typename T, int NW, int TC, int RPT, int CPT, int WPE>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_tile(const LpDesc* __restrict__ desc,
                                                   const double* __restrict__ pool,
                                                   const int8_t* __restrict__ spool,
                                                   const int32_t* __restrict__ idx, int32_t count,
@@ -471,17 +505,20 @@ __global__ __launch_bounds__(NW * 64) void k_tile(const LpDesc* __restrict__ des
   __shared__ T colbuf[Tab::RCAP], rhsbuf[Tab::RCAP], rowbuf[Tab::CCAP], scal[2];
   __shared__ int bs[Tab::RCAP], nb[Tab::CCAP];
   __shared__ uint8_t redund[Tab::RCAP], target[Tab::RCAP + Tab::CCAP];
-  __shared__ RedSlot<T> slots[2 * NW];
+  __shared__ Decision<T> slots[2];
   __shared__ double s_am[NW];
   __shared__ int s_q;
   Tab t;
   t.s_am = s_am;
   t.tid = threadIdx.x;
+  t.lane = threadIdx.x & 63;
+  t.wv = uniform((int)(threadIdx.x >> 6));
   t.gr = threadIdx.x / TC;
   t.gc = threadIdx.x - t.gr * TC;
+  t.phase = 0;
   t.colbuf = colbuf; t.rhsbuf = rhsbuf; t.rowbuf = rowbuf; t.scal = scal;
   t.bs = bs; t.nb = nb; t.redund = redund; t.target = target;
-  t.red.init(slots);
+  t.slots = slots;
   if (count_dev) count = *count_dev;
   for (;;) {
     __syncthreads();
@@ -506,20 +543,22 @@ __global__ __launch_bounds__(NW * 64) void k_tile(const LpDesc* __restrict__ des
 // ---------------------------------------------------------------------------------
 // variant table: (waves per LP, TC, RPT, CPT); capacity (64*NW/TC*RPT) x (TC*CPT)
 // (the autotuner measures every variant that holds a shape; fewer waves per LP mean fewer
-// cross-wave reductions per pivot, more waves mean fewer registers per thread)
-#define BLPG_TILE_VARIANTS(X) \
-  X(0, 1, 8, 4, 5)            \
-  X(1, 1, 8, 7, 7)            \
-  X(2, 2, 16, 8, 7)           \
-  X(3, 2, 16, 10, 6)          \
-  X(4, 4, 16, 7, 7)           \
-  X(5, 8, 16, 6, 9)           \
-  X(6, 8, 16, 7, 13)          \
-  X(7, 1, 8, 10, 11)          \
-  X(8, 1, 16, 16, 7)          \
-  X(9, 2, 16, 14, 7)          \
-  X(10, 4, 32, 14, 4)         \
-  X(11, 8, 16, 4, 7)
+// barriers per pivot, more waves mean fewer registers per thread).  The last two columns
+// are the waves-per-SIMD floors given to the register allocator for FP32 / FP64: set
+// where a few bytes of spill buy one more resident workgroup per CU.
+#define BLPG_TILE_VARIANTS(X)        \
+  X(0, 1, 8, 4, 5, 1, 1)             \
+  X(1, 1, 8, 7, 7, 4, 1)             \
+  X(2, 2, 16, 8, 7, 1, 1)            \
+  X(3, 2, 16, 10, 6, 1, 1)           \
+  X(4, 4, 16, 7, 7, 4, 1)            \
+  X(5, 8, 16, 6, 9, 1, 1)            \
+  X(6, 8, 16, 7, 13, 1, 1)           \
+  X(7, 1, 8, 10, 11, 1, 1)           \
+  X(8, 1, 16, 16, 7, 1, 1)           \
+  X(9, 2, 16, 14, 7, 1, 1)           \
+  X(10, 4, 32, 14, 4, 1, 1)          \
+  X(11, 8, 16, 4, 7, 1, 1)
 
 constexpr int kTileVariants = 12;
 constexpr int kTileDefault = 7;  // static routing uses variants 0..6
@@ -529,7 +568,7 @@ int tile_default_variants() { return kTileDefault; }
 
 TileVariant tile_variant(int, int v) {
   switch (v) {
-#define X(id, nw, tc, rpt, cpt) \
+#define X(id, nw, tc, rpt, cpt, w32, w64) \
   case id: return TileVariant{nw, (nw * 64) / tc, rpt, cpt};
     BLPG_TILE_VARIANTS(X)
 #undef X
@@ -540,8 +579,8 @@ TileVariant tile_variant(int, int v) {
 template <typename T>
 static const void* tile_fn(int v) {
   switch (v) {
-#define X(id, nw, tc, rpt, cpt) \
-  case id: return reinterpret_cast<const void*>(&k_tile<T, nw, tc, rpt, cpt>);
+#define X(id, nw, tc, rpt, cpt, w32, w64) \
+  case id: return reinterpret_cast<const void*>(&k_tile<T, nw, tc, rpt, cpt, sizeof(T) == 4 ? w32 : w64>);
     BLPG_TILE_VARIANTS(X)
 #undef X
     default: return nullptr;

@@ -26,9 +26,10 @@
 
 namespace blpg {
 
+// lexicographic (value, index) order, evaluated without short-circuit branches
 template <typename T>
 __device__ __forceinline__ bool better(T v, int k, T bv, int bk) {
-  return v < bv || (v == bv && k < bk);
+  return (v < bv) | ((v == bv) & (k < bk));
 }
 
 template <typename T>
@@ -53,83 +54,156 @@ __device__ __forceinline__ double uniform(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// DPP lane movement inside 16-lane rows (gfx9 encodings): quad_perm [1,0,3,2] and
-// [2,3,0,1], row_half_mirror (lane i -> 7-i in each 8-lane half), row_mirror (i -> 15-i).
-// Once the lanes of a quad / half-row hold equal keys, the mirrors act as xor-4 / xor-8.
-constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+// ---- (value, index, payload) reductions on packed keys ----
+// A candidate (v, k, p) becomes one integer key whose unsigned order is the lexicographic
+// order of (v, k): v is mapped to an order-preserving integer (sign-magnitude folded;
+// -0 is first turned into +0 so that equal values give equal keys, as `better` compares
+// them), followed by k and p in 16 bits each (indices and variable ids are < 8193; 0xFFFF
+// encodes "none").  FP32: one 64-bit key; FP64: a 64-bit value word plus a 32-bit index
+// word.  Minimising keys is branch-free: a DPP move per word, a compare and selects.
+template <typename T>
+struct KeyOps;
 
-template <int CTRL>
-__device__ __forceinline__ int dpp(int x) {
-  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
-}
-template <int CTRL>
-__device__ __forceinline__ float dpp(float x) {
-  return __int_as_float(dpp<CTRL>(__float_as_int(x)));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = dpp<CTRL>((int)(b & 0xffffffffll));
-  const int hi = dpp<CTRL>((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+template <>
+struct KeyOps<float> {
+  typedef unsigned long long K;
+  static __device__ __forceinline__ K none() { return ~0ull; }
+  static __device__ __forceinline__ K make(float v, int k, int p) {
+    const uint32_t u = __float_as_uint(v + 0.0f);
+    const uint32_t o = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return ((K)o << 32) | ((uint32_t)(k & 0xFFFF) << 16) | (uint32_t)(p & 0xFFFF);
+  }
+  static __device__ __forceinline__ bool lt(K a, K b) { return a < b; }
+  static __device__ __forceinline__ K sel(bool c, K a, K b) { return c ? a : b; }
+  template <int CTRL, int ROWS>
+  static __device__ __forceinline__ K dpp(K a) {
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)a, (int)(uint32_t)a, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(a >> 32), (int)(uint32_t)(a >> 32), CTRL,
+                                               ROWS, 0xF, false);
+    return ((K)(uint32_t)hi << 32) | (uint32_t)lo;
+  }
+  static __device__ __forceinline__ K lane(K a, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)a, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(a >> 32), l);
+    return ((K)hi << 32) | lo;
+  }
+  static __device__ __forceinline__ float value(K a) {
+    const uint32_t o = (uint32_t)(a >> 32);
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+  }
+  static __device__ __forceinline__ uint32_t kp(K a) { return (uint32_t)a; }
+};
+
+template <>
+struct KeyOps<double> {
+  struct K {
+    unsigned long long v;
+    uint32_t kp;
+  };
+  static __device__ __forceinline__ K none() { return K{~0ull, ~0u}; }
+  static __device__ __forceinline__ K make(double v, int k, int p) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v + 0.0);
+    const unsigned long long o = (u >> 63) ? ~u : (u | (1ull << 63));
+    return K{o, ((uint32_t)(k & 0xFFFF) << 16) | (uint32_t)(p & 0xFFFF)};
+  }
+  static __device__ __forceinline__ bool lt(const K& a, const K& b) {
+    return (a.v < b.v) | ((a.v == b.v) & (a.kp < b.kp));
+  }
+  static __device__ __forceinline__ K sel(bool c, const K& a, const K& b) {
+    return K{c ? a.v : b.v, c ? a.kp : b.kp};
+  }
+  template <int CTRL, int ROWS>
+  static __device__ __forceinline__ K dpp(const K& a) {
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)a.v, (int)(uint32_t)a.v, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(a.v >> 32), (int)(uint32_t)(a.v >> 32), CTRL,
+                                               ROWS, 0xF, false);
+    const int q = __builtin_amdgcn_update_dpp((int)a.kp, (int)a.kp, CTRL, ROWS, 0xF, false);
+    return K{((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo, (uint32_t)q};
+  }
+  static __device__ __forceinline__ K lane(const K& a, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)a.v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(a.v >> 32), l);
+    const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)a.kp, l);
+    return K{((unsigned long long)hi << 32) | lo, q};
+  }
+  static __device__ __forceinline__ double value(const K& a) {
+    const unsigned long long o = a.v;
+    return __longlong_as_double((long long)((o >> 63) ? (o & ~(1ull << 63)) : ~o));
+  }
+  static __device__ __forceinline__ uint32_t kp(const K& a) { return a.kp; }
+};
+
+// DPP controls (gfx9): quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror, row_mirror
+// (once quads / half-rows hold equal keys the mirrors act as xor-4 / xor-8), and
+// row_bcast:15 / row_bcast:31 (lane 15 of each row, lane 31, into the following rows).
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140,
+              kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
+// Minimum key over the 64 lanes of a wavefront (all lanes active): four steps leave each
+// 16-lane row with its minimum, two row broadcasts fold the rows into lane 63, whose key
+// is returned to every lane (wave-uniform, in scalar registers).
+template <typename T>
+__device__ __forceinline__ typename KeyOps<T>::K wave_min_key(typename KeyOps<T>::K a) {
+  using KO = KeyOps<T>;
+  typename KO::K b;
+  b = KO::template dpp<kDppXor1, 0xF>(a);       a = KO::sel(KO::lt(b, a), b, a);
+  b = KO::template dpp<kDppXor2, 0xF>(a);       a = KO::sel(KO::lt(b, a), b, a);
+  b = KO::template dpp<kDppHalfMirror, 0xF>(a); a = KO::sel(KO::lt(b, a), b, a);
+  b = KO::template dpp<kDppMirror, 0xF>(a);     a = KO::sel(KO::lt(b, a), b, a);
+  b = KO::template dpp<kDppBcast15, 0xA>(a);    a = KO::sel(KO::lt(b, a), b, a);
+  b = KO::template dpp<kDppBcast31, 0xC>(a);    a = KO::sel(KO::lt(b, a), b, a);
+  return KO::lane(a, 63);
 }
 
-template <int CTRL, typename T>
-__device__ __forceinline__ void dpp_step(T& v, int& k, int& p) {
-  const T ov = dpp<CTRL>(v);
-  const int ok = dpp<CTRL>(k);
-  const int op = dpp<CTRL>(p);
-  const bool b = better(ov, ok, v, k);
-  v = b ? ov : v;
-  k = b ? ok : k;
-  p = b ? op : p;
+template <typename T>
+__device__ __forceinline__ void key_decode(const typename KeyOps<T>::K& a, T& v, int& k, int& p) {
+  using KO = KeyOps<T>;
+  const uint32_t q = KO::kp(a);
+  v = KO::value(a);
+  k = (q >> 16) == 0xFFFFu ? INT_MAX : (int)(q >> 16);
+  p = (q & 0xFFFFu) == 0xFFFFu ? -1 : (int)(q & 0xFFFFu);
 }
 
-__device__ __forceinline__ int lane_read(int x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
-__device__ __forceinline__ float lane_read(float x, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
-}
-__device__ __forceinline__ double lane_read(double x, int lane) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+// Running minimum of one lane's candidates: add(v, k, p) when `take`.
+template <typename T>
+struct ArgMin {
+  using KO = KeyOps<T>;
+  typename KO::K key;
+  __device__ __forceinline__ ArgMin() : key(KO::none()) {}
+  __device__ __forceinline__ void add(bool take, T v, int k, int p) {
+    const typename KO::K c = KO::make(v, k, p);
+    key = KO::sel(take & KO::lt(c, key), c, key);
+  }
+};
+
+// Wavefront minimum of every lane's running minimum, decoded (all lanes active).
+template <typename T>
+__device__ __forceinline__ void wave_finish(const ArgMin<T>& a, T& v, int& k, int& p) {
+  using KO = KeyOps<T>;
+  const typename KO::K m = wave_min_key<T>(a.key);
+  if (KO::kp(m) == ~0u) { v = inf_of<T>(); k = INT_MAX; p = -1; return; }
+  key_decode<T>(m, v, k, p);
 }
 
-// Argmin over the 64 lanes of a wavefront (all lanes active).  Four DPP steps leave every
-// lane of each 16-lane row with the row's winner; the four row winners are read into
-// scalar registers and combined.  Every lane ends with the same (wave-uniform) result.
-// (value, key) is a strict total order, so the result does not depend on the order of
-// combination; `p` is a payload carried with the winning key.
+// Argmin over the 64 lanes of a wavefront (all lanes active) of (v, k) in lexicographic
+// order, carrying payload p; every lane ends with the same (wave-uniform) result.  No
+// candidate: v = +inf, k = INT_MAX, p = -1.
 template <typename T>
 __device__ __forceinline__ void wave_argmin(T& v, int& k, int& p) {
-  dpp_step<kDppXor1>(v, k, p);
-  dpp_step<kDppXor2>(v, k, p);
-  dpp_step<kDppHalfMirror>(v, k, p);
-  dpp_step<kDppMirror>(v, k, p);
-  T bv = lane_read(v, 0);
-  int bk = lane_read(k, 0), bp = lane_read(p, 0);
-#pragma unroll
-  for (int row = 1; row < 4; ++row) {
-    const T rv = lane_read(v, 16 * row);
-    const int rk = lane_read(k, 16 * row), rp = lane_read(p, 16 * row);
-    const bool b = better(rv, rk, bv, bk);
-    bv = b ? rv : bv;
-    bk = b ? rk : bk;
-    bp = b ? rp : bp;
-  }
-  v = bv;
-  k = bk;
-  p = bp;
+  using KO = KeyOps<T>;
+  const bool real = k != INT_MAX;
+  typename KO::K a = KO::sel(real, KO::make(v, k, p), KO::none());
+  a = wave_min_key<T>(a);
+  if (KO::kp(a) == ~0u) { v = inf_of<T>(); k = INT_MAX; p = -1; return; }
+  key_decode<T>(a, v, k, p);
 }
 
 template <typename T>
 struct RedSlot {
-  T v;
-  int k, p;
+  typename KeyOps<T>::K key;
 };
 
-// Workgroup argmin: wavefront butterfly, then the NW per-wave winners through LDS.
+// Workgroup argmin: wavefront minimum, then the NW per-wave minima through LDS.
 // Slots are double-buffered so one barrier per reduction suffices.
 template <typename T, int NW>
 struct Reducer {
@@ -137,24 +211,25 @@ struct Reducer {
   int phase;
   __device__ void init(RedSlot<T>* s) { slots = s; phase = 0; }
   __device__ __forceinline__ void argmin(T& v, int& k, int& p) {
-    wave_argmin(v, k, p);
+    using KO = KeyOps<T>;
+    const bool real = k != INT_MAX;
+    typename KO::K a = KO::sel(real, KO::make(v, k, p), KO::none());
+    a = wave_min_key<T>(a);
     if constexpr (NW > 1) {
       RedSlot<T>* s = slots + phase * NW;
       const int w = threadIdx.x >> 6;
-      if ((threadIdx.x & 63) == 0) { s[w].v = v; s[w].k = k; s[w].p = p; }
+      if ((threadIdx.x & 63) == 0) s[w].key = a;
       __syncthreads();
-      v = s[0].v; k = s[0].k; p = s[0].p;
+      a = s[0].key;
 #pragma unroll
       for (int q = 1; q < NW; ++q) {
-        const T qv = s[q].v;
-        const int qk = s[q].k, qp = s[q].p;
-        const bool b = better(qv, qk, v, k);
-        v = b ? qv : v;
-        k = b ? qk : k;
-        p = b ? qp : p;
+        const typename KO::K b = s[q].key;
+        a = KO::sel(KO::lt(b, a), b, a);
       }
       phase ^= 1;
     }
+    if (KO::kp(a) == ~0u) { v = inf_of<T>(); k = INT_MAX; p = -1; }
+    else key_decode<T>(a, v, k, p);
     v = uniform(v);
     k = uniform(k);
     p = uniform(p);
@@ -214,6 +289,7 @@ __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* 
   int stage = warm ? kStCrash : (sense ? kStEq : kStAux);
   int ei = 0, deg = 0, ci = 0;
   for (;;) {
+    t.loop_fence();
     int pr = -1, ps = -1;
     bool kill_after = false, count_p1 = false;
     if (stage == kStCrash) {
@@ -304,14 +380,19 @@ __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* 
         if (it >= maxit) { status = kStatusIterLimit; break; }
         pr = rx; ps = j; kill_after = true; count_p1 = true;
       } else {
+        // x0 left the basis during phase I: its column goes (through the shared kill site
+        // below -- a second inlined kill costs the whole kernel registers)
         const int j = t.find_nonbasic(n + m);
-        if (j >= 0) t.kill_col(j);
-        continue;
+        if (j < 0) continue;
+        ps = j;
+        kill_after = true;
       }
     }
-    t.pivot(pr, ps);
-    ++it;
-    if (count_p1) ++p1;
+    if (pr >= 0) {
+      t.pivot(pr, ps);
+      ++it;
+      if (count_p1) ++p1;
+    }
     if (kill_after) t.kill_col(ps);
   }
 }
