@@ -45,7 +45,11 @@ bool finite_all(const double* p, int64_t n) {
 
 int grow(blpg_ctx* ctx, DevBuf& b, size_t bytes) {
   if (b.bytes >= bytes) return BLPG_OK;
-  if (b.p) (void)hipFree(b.p);
+  ++ctx->gen;  // captured graphs that hold the old buffer must be captured again
+  if (b.p) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b.p);
+  }
   b.p = nullptr;
   b.bytes = 0;
   if (hipMalloc(&b.p, bytes) != hipSuccess)
@@ -60,6 +64,13 @@ int persistent_grid(const blpg_ctx* ctx, int per_cu, int64_t count) {
 }
 
 }  // namespace
+
+static void drop_graph(blpg_batch* B) {
+  if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->gexec = nullptr;
+  B->graph = nullptr;
+}
 
 // ------------------------------------------------------------------------------------
 extern "C" {
@@ -360,6 +371,11 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
       blpg_batch_free(ctx, B);
       return fail(ctx, BLPG_E_HIP, "hipEventCreate failed");
     }
+  for (int i = 0; i < 16; ++i)
+    if (hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming) != hipSuccess) {
+      blpg_batch_free(ctx, B);
+      return fail(ctx, BLPG_E_HIP, "hipEventCreate failed");
+    }
   B->info.nbins = nb;
   for (int i = 0; i < nb; ++i) {
     B->info.kernel[i] = B->bins[i].kernel;
@@ -375,9 +391,12 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
 void blpg_batch_free(blpg_ctx* ctx, blpg_batch* B) {
   if (!B) return;
   if (ctx) (void)hipSetDevice(ctx->device);
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(B);
   for (int i = 0; i < 96; ++i)
     if (B->ev[i]) (void)hipEventDestroy(B->ev[i]);
-  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  for (int i = 0; i < 16; ++i)
+    if (B->join[i]) (void)hipEventDestroy(B->join[i]);
   ws_free_rows(B->ws);
   if (B->h_res) (void)hipHostFree(B->h_res);
   if (B->warm_mem) (void)hipFree(B->warm_mem);
@@ -452,32 +471,37 @@ static int launch_bin(blpg_ctx* ctx, blpg_batch* B, const Bin& b, const BinPlan&
   return BLPG_OK;
 }
 
-// Event slots of a batch: ev[0] start; per bin i: ev[1+4i] solve start, ev[2+4i] solve end,
-// ev[3+4i] verify end, ev[4+4i] fallback end; ev[65] end of everything; ev[66], ev[67]
-// bracket the warm-start inference.
-static constexpr int kEvEnd = 65;
-static constexpr int kEvWsStart = 66, kEvWsEnd = 67;  // learned warm start inference
+// Launch plans of every bin (grids, HBM workspace offsets, check-pass sizes), with the
+// context workspaces grown to fit: the HBM slabs of streamed bins, of the FP64 fallback,
+// and the LU slabs of the check (bins run concurrently, so each gets its own region).
+struct SolvePlan {
+  std::vector<BinPlan> plans, fplans;
+  std::vector<Bin> fbins;
+  std::vector<size_t> ws_off, fws_off, vws_off;
+  std::vector<int> vgrid, vgrid2, klds1, klds2;
+  std::vector<int64_t> vstride;
+};
 
-int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
-  if (!ctx || !B) return BLPG_E_INVALID;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  if (B->count == 0) return BLPG_OK;
-  hipStream_t st = ctx->stream;
-  const int precision = ctx->opt.precision;
-  B->precision = precision;
-  int32_t* q = B->d_counters;
+static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
   const int nb = (int)B->bins.size();
-  double eps_d, eps_p, tol, kkt;
-  resolve(ctx->opt, eps_d, eps_p, tol, kkt);
+  const int precision = ctx->opt.precision;
   const bool do_verify = ctx->opt.verify || precision == BLPG_FP32;
   const bool fp32 = precision == BLPG_FP32;
-
-  // ---- launch plans; HBM workspaces carved per bin (bins run concurrently) ----
-  std::vector<BinPlan> plans(nb), fplans(nb);
-  std::vector<Bin> fbins(nb);
-  std::vector<size_t> ws_off(nb, 0), fws_off(nb, 0), vws_off(nb, 0);
-  std::vector<int> vgrid(nb, 1), vgrid2(nb, 1), klds1(nb, 64), klds2(nb, 64);
-  std::vector<int64_t> vstride(nb, 0);
+  P.plans.assign(nb, BinPlan{}); P.fplans.assign(nb, BinPlan{}); P.fbins.assign(nb, Bin{});
+  P.ws_off.assign(nb, 0); P.fws_off.assign(nb, 0); P.vws_off.assign(nb, 0);
+  P.vgrid.assign(nb, 1); P.vgrid2.assign(nb, 1); P.klds1.assign(nb, 64); P.klds2.assign(nb, 64);
+  P.vstride.assign(nb, 0);
+  std::vector<BinPlan>& plans = P.plans;
+  std::vector<BinPlan>& fplans = P.fplans;
+  std::vector<Bin>& fbins = P.fbins;
+  std::vector<size_t>& ws_off = P.ws_off;
+  std::vector<size_t>& fws_off = P.fws_off;
+  std::vector<size_t>& vws_off = P.vws_off;
+  std::vector<int>& vgrid = P.vgrid;
+  std::vector<int>& vgrid2 = P.vgrid2;
+  std::vector<int>& klds1 = P.klds1;
+  std::vector<int>& klds2 = P.klds2;
+  std::vector<int64_t>& vstride = P.vstride;
   size_t ws_total = 0, fws_total = 0, vws_total = 0;
   for (int i = 0; i < nb; ++i) {
     const Bin& b = B->bins[i];
@@ -518,17 +542,62 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
   if (ws_total) { const int rc = grow(ctx, ctx->ws, ws_total); if (rc) return rc; }
   if (fws_total) { const int rc = grow(ctx, ctx->ws2, fws_total); if (rc) return rc; }
   if (vws_total) { const int rc = grow(ctx, ctx->vws, vws_total); if (rc) return rc; }
+  return BLPG_OK;
+}
 
-  HIPCHK(ctx, hipEventRecord(B->ev[0], st));
+static int plan_workspaces(blpg_ctx* ctx, blpg_batch* B) {
+  SolvePlan P;
+  return plan_solve(ctx, B, P);
+}
+
+// Event slots of a batch: ev[0] start; per bin i: ev[1+4i] solve start, ev[2+4i] solve end,
+// ev[3+4i] verify end, ev[4+4i] fallback end; ev[65] end of everything; ev[66], ev[67]
+// bracket the warm-start inference.
+static constexpr int kEvEnd = 65;
+static constexpr int kEvWsStart = 66, kEvWsEnd = 67;  // learned warm start inference
+
+// Enqueues one solve of the batch on the context stream (and the aux streams it forks
+// to).  With `timing` every stage is bracketed by timing events; without, only the
+// fork / join events are recorded, so the sequence can be captured into a hipGraph.
+// Workspaces are grown first: nothing in the enqueued sequence allocates.
+static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
+  hipStream_t st = ctx->stream;
+  const int precision = ctx->opt.precision;
+  B->precision = precision;
+  int32_t* q = B->d_counters;
+  const int nb = (int)B->bins.size();
+  double eps_d, eps_p, tol, kkt;
+  resolve(ctx->opt, eps_d, eps_p, tol, kkt);
+  const bool do_verify = ctx->opt.verify || precision == BLPG_FP32;
+  const bool fp32 = precision == BLPG_FP32;
+
+  SolvePlan P;
+  {
+    const int rc = plan_solve(ctx, B, P);
+    if (rc) return rc;
+  }
+  std::vector<BinPlan>& plans = P.plans;
+  std::vector<BinPlan>& fplans = P.fplans;
+  std::vector<Bin>& fbins = P.fbins;
+  std::vector<size_t>& ws_off = P.ws_off;
+  std::vector<size_t>& fws_off = P.fws_off;
+  std::vector<size_t>& vws_off = P.vws_off;
+  std::vector<int>& vgrid = P.vgrid;
+  std::vector<int>& vgrid2 = P.vgrid2;
+  std::vector<int>& klds1 = P.klds1;
+  std::vector<int>& klds2 = P.klds2;
+  std::vector<int64_t>& vstride = P.vstride;
+
+  if (timing) HIPCHK(ctx, hipEventRecord(B->ev[0], st));
   HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
   HIPCHK(ctx, launch_fill_i32(q, 0, 128, st));
   // learned warm start: crash lists of the family LPs, before any solve kernel
-  HIPCHK(ctx, hipEventRecord(B->ev[kEvWsStart], st));
+  if (timing) HIPCHK(ctx, hipEventRecord(B->ev[kEvWsStart], st));
   if (B->model) {
     const int rc = ws_run_batch(ctx, B, st);
     if (rc) return rc;
   }
-  HIPCHK(ctx, hipEventRecord(B->ev[kEvWsEnd], st));
+  if (timing) HIPCHK(ctx, hipEventRecord(B->ev[kEvWsEnd], st));
   // ---- fork: each size bin runs solve -> check -> fallback on one of kAux+1 streams;
   //      bins are placed longest first on the least loaded stream (cost ~ LPs * R * C) ----
   HIPCHK(ctx, hipEventRecord(ctx->fork, st));
@@ -560,11 +629,11 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
     hipStream_t s = lane_of[i] == 0 ? st : ctx->aux[lane_of[i] - 1];
     if (s != st && !used[lane_of[i]]) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->fork, 0));
     used[lane_of[i]] = true;
-    HIPCHK(ctx, hipEventRecord(B->ev[1 + 4 * i], s));
+    if (timing) HIPCHK(ctx, hipEventRecord(B->ev[1 + 4 * i], s));
     int rc = launch_bin(ctx, B, b, plans[i], precision, b.d_idx, cnt, nullptr, qb,
                         (char*)ctx->ws.p + ws_off[i], B->d_warm, s);
     if (rc) return rc;
-    HIPCHK(ctx, hipEventRecord(B->ev[2 + 4 * i], s));
+    if (timing) HIPCHK(ctx, hipEventRecord(B->ev[2 + 4 * i], s));
     if (do_verify) {
       VerifyLaunch V{};
       V.desc = B->d_desc; V.pool = B->d_pool; V.spool = B->d_spool;
@@ -599,7 +668,7 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
       V.grid = vgrid2[i];
       V.big_list = nullptr;
       V.big_count = nullptr;
-      HIPCHK(ctx, hipEventRecord(B->ev[3 + 4 * i], s));
+      if (timing) HIPCHK(ctx, hipEventRecord(B->ev[3 + 4 * i], s));
       if (fp32) {
         // FP64 re-solve of the LPs of this bin whose FP32 result failed; the count is
         // read on the device, so no host synchronisation is needed.
@@ -616,15 +685,57 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
         V2.fail_count = nullptr;
         HIPCHK(ctx, launch_verify(V2, s));
       }
-      HIPCHK(ctx, hipEventRecord(B->ev[4 + 4 * i], s));
+      if (timing) HIPCHK(ctx, hipEventRecord(B->ev[4 + 4 * i], s));
     }
     if (s != st) {
-      HIPCHK(ctx, hipEventRecord(B->ev[2 + 4 * i + (do_verify ? 2 : 0)], s));
-      HIPCHK(ctx, hipStreamWaitEvent(st, B->ev[2 + 4 * i + (do_verify ? 2 : 0)], 0));
+      HIPCHK(ctx, hipEventRecord(B->join[i], s));
+      HIPCHK(ctx, hipStreamWaitEvent(st, B->join[i], 0));
     }
   }
-  HIPCHK(ctx, hipEventRecord(B->ev[kEvEnd], st));
+  if (timing) {
+    HIPCHK(ctx, hipEventRecord(B->ev[kEvEnd], st));
+    B->nev = kEvEnd + 1;
+  }
+  B->timed = timing;
+  return BLPG_OK;
+}
+
+int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
+  if (!ctx || !B) return BLPG_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (B->count == 0) return BLPG_OK;
+  if (!ctx->opt.use_graph) return enqueue_solve(ctx, B, true);
+  // hipGraph path: the whole sequence (clears, warm-start network, every bin's solve /
+  // check / fallback on its stream, joins) is captured once per batch and replayed; it is
+  // captured again when options, the attached model or the context's workspaces change.
+  blpg_options key = ctx->opt;
+  key.use_graph = 1;
+  const bool stale = !B->gexec || std::memcmp(&key, &B->g_opt, sizeof key) != 0 ||
+                     B->g_model != B->model || B->g_gen != ctx->gen;
+  if (stale) {
+    drop_graph(B);
+    int rc = plan_workspaces(ctx, B);  // allocations happen outside the capture
+    if (rc) return rc;
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_solve(ctx, B, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess || !g) return fail(ctx, BLPG_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    B->graph = g;
+    if (hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+      drop_graph(B);
+      return fail(ctx, BLPG_E_HIP, "hipGraphInstantiate failed");
+    }
+    B->g_opt = key;
+    B->g_model = B->model;
+    B->g_gen = ctx->gen;
+  }
+  HIPCHK(ctx, hipEventRecord(B->ev[0], ctx->stream));
+  HIPCHK(ctx, hipGraphLaunch(B->gexec, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(B->ev[kEvEnd], ctx->stream));
   B->nev = kEvEnd + 1;
+  B->timed = false;
   return BLPG_OK;
 }
 
@@ -641,6 +752,15 @@ int blpg_batch_info_get(blpg_ctx* ctx, const blpg_batch* B, blpg_batch_info* inf
   const bool checked = ctx->opt.verify || B->precision == BLPG_FP32;
   float ms = 0;
   info->verify_ms = info->fallback_ms = 0;
+  info->warmstart_ms = 0;
+  info->total_ms = 0;
+  if (B->nev > 0 && hipEventElapsedTime(&ms, B->ev[0], B->ev[kEvEnd]) == hipSuccess)
+    info->total_ms = ms;
+  if (!B->timed) {  // graph replay: only the whole solve is timed
+    for (int i = 0; i < nb; ++i) info->kernel_ms[i] = -1.0;
+    info->verify_ms = info->fallback_ms = info->warmstart_ms = -1.0;
+    return BLPG_OK;
+  }
   for (int i = 0; i < nb; ++i) {
     if (hipEventElapsedTime(&ms, B->ev[1 + 4 * i], B->ev[2 + 4 * i]) == hipSuccess)
       info->kernel_ms[i] = ms;
@@ -651,9 +771,6 @@ int blpg_batch_info_get(blpg_ctx* ctx, const blpg_batch* B, blpg_batch_info* inf
         info->fallback_ms = std::max(info->fallback_ms, (double)ms);
     }
   }
-  if (B->nev > 0 && hipEventElapsedTime(&ms, B->ev[0], B->ev[kEvEnd]) == hipSuccess)
-    info->total_ms = ms;
-  info->warmstart_ms = 0;
   if (B->nev > 0 && B->model &&
       hipEventElapsedTime(&ms, B->ev[kEvWsStart], B->ev[kEvWsEnd]) == hipSuccess)
     info->warmstart_ms = ms;
@@ -781,6 +898,7 @@ int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps
   const int prec = ctx->opt.precision;
   const blpg_options saved = ctx->opt;
   ctx->opt.kernel = BLPG_KERNEL_AUTO;
+  ctx->opt.use_graph = 0;  // per-bin kernel times come from the timed (eager) path
   std::map<std::pair<int, int>, std::vector<blpg_lp>> groups;
   for (int64_t k = 0; k < count; ++k) {
     blpg_lp L = lps[k];

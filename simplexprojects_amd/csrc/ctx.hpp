@@ -61,6 +61,7 @@ struct blpg_ctx {
   static constexpr int kAux = 3;
   hipStream_t aux[kAux] = {};
   hipEvent_t fork = nullptr;
+  int64_t gen = 0;  // bumped whenever a workspace is reallocated (captured graphs go stale)
 };
 
 struct blpg_batch {
@@ -92,7 +93,14 @@ struct blpg_batch {
   std::vector<int64_t> x_off, basis_off;
   blpg_batch_info info{};
   hipEvent_t ev[96] = {};
+  hipEvent_t join[16] = {};  // per-bin join of an aux stream back into the context stream
   int nev = 0;
+  bool timed = false;        // last solve recorded per-stage timing events
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  blpg_options g_opt{};      // options / model / workspace generation the graph holds
+  blpg_ws_model* g_model = nullptr;
+  int64_t g_gen = -1;
 };
 
 

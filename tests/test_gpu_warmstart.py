@@ -95,7 +95,7 @@ def test_training_and_warm_start_gain():
 
     test = sum((family(f, 512, 64) for f in range(nf)), [])
     cold = S64.solve(test)
-    S32 = sa.Solver(precision="fp32")
+    S32 = sa.Solver(precision="fp32", use_graph=False)
     batch = S32.upload(test)
     batch.set_warmstart(ws)
     batch.solve()
@@ -103,6 +103,13 @@ def test_training_and_warm_start_gain():
     warm = batch.download()
     info = batch.info()
     assert info["warmstart_ms"] > 0
+    # the same solve replayed through a captured graph (network included) is identical
+    S32.set(use_graph=1)
+    for _ in range(3):
+        batch.solve()
+        g = batch.fetch()
+        assert np.array_equal(g.pivots, warm.pivots) and np.array_equal(g.objective, warm.objective)
+    S32.set(use_graph=0)
     assert np.array_equal(warm.status, cold.status)
     rel = np.abs(warm.objective - cold.objective) / np.maximum(1, np.abs(cold.objective))
     assert rel.max() <= 1e-6
