@@ -182,6 +182,47 @@ def cpu_baseline(lps, threads):
     return r, dt
 
 
+def serving_leg(sa, tune32=None, seconds=0.5, rate=50000.0):
+    """Serving loop: Poisson arrivals at `rate` LPs/s of single LPs, shapes uniform over
+    27x32, 50x48, 74x83, 105x103, 205x203 (generators as the first two workloads, seed 5),
+    FP32 tableau + FP64 check; latency counted from each scheduled arrival."""
+    shapes = [(27, 32), (50, 48), (74, 83), (105, 103), (205, 203)]
+    pools = [sa.gen_config(5, 5, m, n, 512, shape_idx=i, batch_count=1024)
+             for i, (m, n) in enumerate(shapes)]
+    rng = np.random.default_rng(5)
+    count = int(rate * seconds)
+    which = rng.integers(0, len(shapes), count)
+    slot = rng.integers(0, 512, count)
+    lps = [pools[w][k] for w, k in zip(which, slot)]
+    arrival = np.cumsum(rng.exponential(1e6 / rate, count))
+    S = sa.Solver(precision="fp32")
+    if tune32:
+        S.tune_import(tune32)
+    srv = sa.Server(S, max_batch=2048, max_wait_us=200, precision="fp32")
+    srv.replay([p[0] for p in pools] * 2, np.arange(10) * 10.0)  # arenas + graphs built
+    base = srv.stats()
+    t0 = time.perf_counter()
+    res = srv.replay(lps, arrival)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    S.close()
+    ref = sa.Solver(precision="fp64").solve(lps[:4096])
+    opt = ref.status == sa.OPTIMAL
+    rel = (np.abs(res["objective"][:4096][opt] - ref.objective[opt]) /
+           np.maximum(1, np.abs(ref.objective[opt])))
+    lat = res["latency_us"]
+    return {"workload": f"Poisson {rate:.0f} LPs/s for {seconds} s, shapes uniform over "
+                        "27x32, 50x48, 74x83, 105x103, 205x203, seed 5",
+            "offered": count, "completed": int((res["status"] != sa.UNDECIDED).sum()),
+            "lps_per_s": count / wall, "wall_s": wall,
+            "p50_us": float(np.percentile(lat, 50)), "p99_us": float(np.percentile(lat, 99)),
+            "max_us": float(lat.max()), "batches": int(st["batches"] - base["batches"]),
+            "mean_batch": (count / max(1, st["batches"] - base["batches"])),
+            "status_agreement_fp64": float(np.mean(res["status"][:4096] == ref.status)),
+            "max_rel_obj_err": float(rel.max()) if opt.any() else 0.0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +230,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="static routing (no autotune pass)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-warmstart", action="store_true",
                     help="skip the learned warm start leg (parametric families)")
     args = ap.parse_args()
@@ -244,6 +286,8 @@ def main():
     wsleg = (None if args.no_warmstart else
              warmstart_leg(sa, args.steps, args.warmup, do_tune=not args.no_tune))
 
+    serve = None if args.no_serve else serving_leg(sa, tune32)
+
     line = {
         "metric": METRIC, "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t32 * 1e3, "higher_is_better": True,
@@ -272,6 +316,7 @@ def main():
         "pivots_mean": float(np.mean(r64.pivots)),
         "tuning": tuning,
         "warmstart": wsleg,
+        "serving": serve,
     }
     print(json.dumps(line))
 

@@ -220,6 +220,51 @@ int blpg_ws_norm(blpg_ctx *ctx, const blpg_ws_model *model, int32_t family, floa
 int blpg_ws_gradient(blpg_ctx *ctx, blpg_ws_model *model, const blpg_lp *lps,
                      const int32_t *bases, int64_t count, float *grad, float *loss);
 
+/* ---- serving (no reference counterpart: the reference solves one batch per call) ----
+ * A server admits single LPs, keeps them resident in device arenas (one per LP shape,
+ * fixed slots, a device-side LP count), forms batches from its queue -- a batch launches
+ * when it is full or when its oldest request has waited max_wait_us -- and replays each
+ * arena's captured hipGraph for any batch size.  Requests are admitted against an HBM
+ * budget: a submit that would exceed it returns BLPG_E_BUDGET (retry after polling).
+ * A worker thread owned by the server does the batching; submit / poll are thread-safe. */
+typedef struct blpg_server blpg_server;
+typedef struct {
+  int64_t hbm_budget;   /* bytes of device memory for arenas and resident requests; 0 = 64 GiB */
+  int32_t max_batch;    /* LPs per launch (arena capacity per shape); 0 = 2048 */
+  int32_t max_wait_us;  /* batching delay bound; 0 = 200 */
+  int32_t precision;    /* BLPG_FP64 or BLPG_FP32 (FP64 check + fallback) */
+  int32_t arenas;       /* arenas per shape (batches of one shape in flight); 0 = 2 */
+} blpg_server_config;
+typedef struct {
+  int64_t ticket;       /* as returned by blpg_server_submit */
+  int32_t status, pivots;
+  double objective;
+  double latency_us;    /* submit (or scheduled arrival, blpg_server_replay) to completion */
+} blpg_server_result;
+typedef struct {
+  int64_t submitted, completed, rejected, batches;
+  double mean_batch;            /* LPs per launch */
+  double p50_us, p99_us, max_us; /* completion latency over all completed requests */
+  int64_t resident_bytes;       /* admitted, not yet polled */
+} blpg_server_stats;
+
+int blpg_server_create(blpg_ctx *ctx, const blpg_server_config *cfg, blpg_server **server);
+void blpg_server_destroy(blpg_server *server);
+/* Copies the LP (shape m x n, '<=' rows only or with senses).  x / basis of the result are
+ * written to the caller's buffers (n doubles, m int32; may be NULL) before the ticket is
+ * returned by poll; the buffers must stay valid until then. */
+int blpg_server_submit(blpg_server *server, const blpg_lp *lp, double *x, int32_t *basis,
+                       int64_t *ticket);
+/* Up to `max` completed results; waits up to wait_us for the first one. */
+int blpg_server_poll(blpg_server *server, blpg_server_result *out, int32_t max, int32_t wait_us,
+                     int32_t *n);
+int blpg_server_stats_get(blpg_server *server, blpg_server_stats *stats);
+/* Load generator: submits lps[k] at arrival_us[k] (microseconds after the call, ascending)
+ * from a pacing thread, retrying admission, and waits until all completed; latencies count
+ * from the scheduled arrival.  Results per LP go to out[k] (optional). */
+int blpg_server_replay(blpg_server *server, const blpg_lp *lps, const double *arrival_us,
+                       int64_t count, blpg_server_result *out);
+
 /* Seeded synthetic LP generators (counter-based; bit-identical on every host).
  * config 1: A ~ U[0,1) 30% zeroed, b ~ U[1,10), c ~ U[0,1);
  * config 2: A ~ U[-1,1) 40% zeroed, b ~ U[0,10) with floor(0.1m) zero rows, c ~ N(0,1),

@@ -77,6 +77,22 @@ class _BatchResults(C.Structure):
                 ("x_offset", C.POINTER(C.c_int64)), ("basis_offset", C.POINTER(C.c_int64))]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_batch", C.c_int32), ("max_wait_us", C.c_int32),
+                ("precision", C.c_int32), ("arenas", C.c_int32)]
+
+
+class _ServerResult(C.Structure):
+    _fields_ = [("ticket", C.c_int64), ("status", C.c_int32), ("pivots", C.c_int32),
+                ("objective", C.c_double), ("latency_us", C.c_double)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+                ("batches", C.c_int64), ("mean_batch", C.c_double), ("p50_us", C.c_double),
+                ("p99_us", C.c_double), ("max_us", C.c_double), ("resident_bytes", C.c_int64)]
+
+
 class _WsConfig(C.Structure):
     _fields_ = [("hidden", C.c_int32), ("steps", C.c_int32), ("lr", C.c_float),
                 ("weight_decay", C.c_float), ("seed", C.c_uint64)]
@@ -90,7 +106,9 @@ ABI_SYMBOLS = [
     "blpg_batch_fetch", "blpg_autotune", "blpg_tune_set", "blpg_tune_export", "blpg_tune_import",
     "blpg_batch_info_get", "blpg_gen_config", "blpg_version", "blpg_ws_create",
     "blpg_ws_destroy", "blpg_ws_train", "blpg_ws_predict", "blpg_batch_set_warmstart",
-    "blpg_ws_params", "blpg_ws_norm", "blpg_ws_gradient",
+    "blpg_ws_params", "blpg_ws_norm", "blpg_ws_gradient", "blpg_server_create",
+    "blpg_server_destroy", "blpg_server_submit", "blpg_server_poll", "blpg_server_stats_get",
+    "blpg_server_replay",
 ]
 
 _lib: Optional[C.CDLL] = None
@@ -133,6 +151,13 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
                                     P(C.c_double), P(C.c_double), P(C.c_int8), P(C.c_int32)]
     lib.blpg_version.restype = C.c_char_p
     F, I32 = P(C.c_float), P(C.c_int32)
+    lib.blpg_server_create.argtypes = [C.c_void_p, P(_ServerConfig), P(C.c_void_p)]
+    lib.blpg_server_destroy.argtypes = [C.c_void_p]
+    lib.blpg_server_submit.argtypes = [C.c_void_p, P(_LP), P(C.c_double), I32, P(C.c_int64)]
+    lib.blpg_server_poll.argtypes = [C.c_void_p, P(_ServerResult), C.c_int32, C.c_int32, I32]
+    lib.blpg_server_stats_get.argtypes = [C.c_void_p, P(_ServerStats)]
+    lib.blpg_server_replay.argtypes = [C.c_void_p, P(_LP), P(C.c_double), C.c_int64,
+                                       P(_ServerResult)]
     lib.blpg_ws_create.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(_WsConfig),
                                    P(C.c_void_p)]
     lib.blpg_ws_destroy.argtypes = [C.c_void_p, C.c_void_p]
@@ -520,6 +545,76 @@ class WarmStart:
     def close(self) -> None:
         if getattr(self, "h", None):
             self.solver.lib.blpg_ws_destroy(self.solver.ctx, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop (blpg_server_*): single LPs in, batches formed on the device side of a
+    queue, replayed through captured hipGraphs, results out with their latency."""
+
+    def __init__(self, solver: Solver, hbm_budget: int = 0, max_batch: int = 0,
+                 max_wait_us: int = 0, precision: str = "fp64", arenas: int = 0):
+        self.solver = solver
+        cfg = _ServerConfig(hbm_budget, max_batch, max_wait_us,
+                            {"fp64": FP64, "fp32": FP32}[precision], arenas)
+        h = C.c_void_p()
+        solver._check(solver.lib.blpg_server_create(solver.ctx, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, lp: LP, want_x: bool = False) -> int:
+        """Returns the ticket; raises BlpgError(E_BUDGET) when admission is refused."""
+        s = _lp_struct(lp)
+        x = np.zeros(lp.n) if want_x else None
+        bs = np.zeros(lp.m, dtype=np.int32) if want_x else None
+        t = C.c_int64()
+        rc = self.solver.lib.blpg_server_submit(self.h, C.byref(s), _dp(x), _i32p(bs), C.byref(t))
+        if rc != 0:
+            raise BlpgError(rc, "blpg_server_submit refused the LP")
+        self._keep[t.value] = (lp, x, bs)
+        return t.value
+
+    def poll(self, max_results: int = 1024, wait_us: int = 0) -> list[dict]:
+        buf = (_ServerResult * max_results)()
+        n = C.c_int32()
+        self.solver._check(self.solver.lib.blpg_server_poll(self.h, buf, max_results, wait_us,
+                                                            C.byref(n)))
+        out = []
+        for i in range(n.value):
+            r = buf[i]
+            lp, x, bs = self._keep.pop(r.ticket, (None, None, None))
+            out.append({"ticket": r.ticket, "status": r.status, "pivots": r.pivots,
+                        "objective": r.objective, "latency_us": r.latency_us, "x": x,
+                        "basis": bs})
+        return out
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        self.solver._check(self.solver.lib.blpg_server_stats_get(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
+
+    def replay(self, lps: Sequence[LP], arrival_us: np.ndarray):
+        """Submit lps[k] at arrival_us[k] (from now) and wait for all; per-LP results."""
+        lps = list(lps)
+        arr = (_LP * len(lps))(*[_lp_struct(lp) for lp in lps])
+        t = np.ascontiguousarray(arrival_us, dtype=np.float64)
+        out = (_ServerResult * len(lps))()
+        self.solver._check(self.solver.lib.blpg_server_replay(self.h, arr, _dp(t), len(lps), out))
+        dt = np.dtype([("ticket", np.int64), ("status", np.int32), ("pivots", np.int32),
+                       ("objective", np.float64), ("latency_us", np.float64)])
+        assert dt.itemsize == C.sizeof(_ServerResult)
+        return np.frombuffer((C.c_char * (C.sizeof(_ServerResult) * len(lps))).from_address(
+            C.addressof(out)), dtype=dt, count=len(lps)).copy()
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.solver.lib.blpg_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

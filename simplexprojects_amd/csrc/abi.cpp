@@ -630,14 +630,16 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
     if (s != st && !used[lane_of[i]]) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->fork, 0));
     used[lane_of[i]] = true;
     if (timing) HIPCHK(ctx, hipEventRecord(B->ev[1 + 4 * i], s));
-    int rc = launch_bin(ctx, B, b, plans[i], precision, b.d_idx, cnt, nullptr, qb,
+    // arena batches (serving) take their LP count from the device, so one captured graph
+    // serves every batch size up to the capacity
+    int rc = launch_bin(ctx, B, b, plans[i], precision, b.d_idx, cnt, B->d_count, qb,
                         (char*)ctx->ws.p + ws_off[i], B->d_warm, s);
     if (rc) return rc;
     if (timing) HIPCHK(ctx, hipEventRecord(B->ev[2 + 4 * i], s));
     if (do_verify) {
       VerifyLaunch V{};
       V.desc = B->d_desc; V.pool = B->d_pool; V.spool = B->d_spool;
-      V.idx = b.d_idx; V.count = cnt;
+      V.idx = b.d_idx; V.count = cnt; V.count_dev = B->d_count;
       V.out = B->out; V.chk = B->chk; V.tol = kkt;
       V.fp32_source = fp32;
       V.mcap = b.rcap - 2;
@@ -824,6 +826,60 @@ int blpg_batch_fetch(blpg_ctx* ctx, blpg_batch* B, blpg_batch_results* R) {
   R->basis_offset = B->basis_off.data();
   return BLPG_OK;
 }
+
+}  // extern "C"
+
+// Results of slots 0..k-1 only (arena batches): the same host layout as blpg_batch_fetch,
+// filled with one small copy per array instead of the whole span.
+int blpg::batch_fetch_prefix(blpg_ctx* ctx, blpg_batch* B, int64_t k, blpg_batch_results* R) {
+  if (!ctx || !B || !R || k < 0 || k > B->count) return BLPG_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int64_t n = B->count;
+  std::memset(R, 0, sizeof *R);
+  R->count = k;
+  if (!B->h_res && hipHostMalloc((void**)&B->h_res, B->res_bytes, hipHostMallocDefault) != hipSuccess) {
+    B->h_res = nullptr;
+    return fail(ctx, BLPG_E_NOMEM, "hipHostMalloc(%zu) for results failed", B->res_bytes);
+  }
+  hipStream_t st = ctx->stream;
+  char* h = B->h_res;
+  char* d = (char*)B->mem + B->res_off;
+  auto cp = [&](const void* dptr, size_t bytes) -> hipError_t {
+    const size_t off = (const char*)dptr - d;
+    return bytes ? hipMemcpyAsync(h + off, dptr, bytes, hipMemcpyDeviceToHost, st) : hipSuccess;
+  };
+  if (k > 0) {
+    const LpDesc& last = B->desc[k - 1];
+    for (int a = 0; a < 6; ++a) HIPCHK(ctx, cp(B->out.status + a * n, sizeof(int32_t) * k));
+    for (int a = 0; a < 5; ++a) HIPCHK(ctx, cp(B->out.obj + a * n, sizeof(double) * k));
+    HIPCHK(ctx, cp(B->out.x, sizeof(double) * (last.x + last.n)));
+    HIPCHK(ctx, cp(B->out.basis, sizeof(int32_t) * (last.basis + last.m)));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  int32_t* i32 = (int32_t*)h;
+  double* f64 = (double*)(h + ((char*)B->out.obj - (char*)B->out.status));
+  const bool verified = ctx->opt.verify || B->precision == BLPG_FP32;
+  for (int64_t q = 0; q < k; ++q) {
+    const int ok = i32[4 * n + q];
+    if (verified && ok == 1) f64[q] = f64[2 * n + q];
+    if (!verified) { f64[3 * n + q] = -1.0; f64[4 * n + q] = -1.0; }
+    if (B->precision == BLPG_FP32 && verified && ok == 0 && i32[q] == BLPG_OPTIMAL) i32[q] = BLPG_NUMERIC;
+  }
+  R->status = i32;
+  R->pivots = i32 + n;
+  R->phase1_pivots = i32 + 2 * n;
+  R->fallback_used = i32 + 5 * n;
+  R->objective = f64;
+  R->kkt_primal = f64 + 3 * n;
+  R->kkt_dual = f64 + 4 * n;
+  R->x = (const double*)(h + ((char*)B->out.x - (char*)B->out.status));
+  R->basis = (const int32_t*)(h + ((char*)B->out.basis - (char*)B->out.status));
+  R->x_offset = B->x_off.data();
+  R->basis_offset = B->basis_off.data();
+  return BLPG_OK;
+}
+
+extern "C" {
 
 int blpg_batch_download(blpg_ctx* ctx, blpg_batch* B, blpg_result* out) {
   if (!ctx || !B || (B->count > 0 && !out)) return BLPG_E_INVALID;

@@ -86,6 +86,7 @@ struct blpg_batch {
   int32_t* d_all = nullptr;       // 0..count-1
   int32_t* d_fail = nullptr;      // LP ids failing the check
   int32_t* d_big = nullptr;       // LP ids whose check needs the larger LDS block
+  int32_t* d_count = nullptr;     // arena batches: number of LPs (slots 0..count-1) to solve
   int32_t* d_counters = nullptr;  // [0..nbins) queues, then verify/fallback queues and counts
   std::vector<blpg::Bin> bins;
   size_t res_off = 0, res_bytes = 0;  // device span of the results (one DMA)
@@ -115,6 +116,9 @@ inline int fail(blpg_ctx* ctx, int code, const char* fmt, ...) {
   if (ctx) ctx->err = buf;
   return code;
 }
+
+// results of slots 0..k-1 of a batch (abi.cpp; the serving arenas use it)
+int batch_fetch_prefix(blpg_ctx* ctx, blpg_batch* B, int64_t k, blpg_batch_results* R);
 
 // warm-start inference of a batch's attached model (warmstart.cpp)
 int ws_run_batch(blpg_ctx* ctx, blpg_batch* B, hipStream_t st);

@@ -1,0 +1,470 @@
+// serve.cpp -- serving loop: single-LP requests, device-resident arenas, batches formed
+// from a queue and launched through captured hipGraphs.  No reference counterpart (the
+// reference solves one batch per ComputeLP call, GPU_Tester.cu:244-259).
+//
+// Per LP shape (m, n, with / without senses) the server keeps `arenas` device batches of
+// `max_batch` slots each (fixed pool offsets per slot, a device-side LP count), each with
+// its own context and stream.  The worker thread
+//   - completes finished arenas (hipStreamQuery), copying only the used slots' results;
+//   - launches a free arena when its shape's queue is full, when the oldest request has
+//     waited max_wait_us, or when no arena of that shape is busy: the queued LPs are
+//     staged into a pinned mirror of the arena's pool, copied with one H2D per pool, the
+//     count is written to the device, and the arena's graph is replayed.
+// Admission: arena bytes + bytes of admitted, unpolled requests <= hbm_budget.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace blpg;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Req {
+  int64_t ticket = 0;
+  int m = 0, n = 0;
+  std::vector<double> data;  // A (m*n) | b (m) | c (n)
+  std::vector<int8_t> sense;
+  double* x = nullptr;
+  int32_t* basis = nullptr;
+  Clock::time_point t0;
+  int64_t bytes = 0;
+};
+
+struct Arena {
+  blpg_ctx* ctx = nullptr;
+  blpg_batch* B = nullptr;
+  double* h_pool = nullptr;  // pinned mirror of the device pool
+  int8_t* h_sense = nullptr; // pinned mirror of the sense pool
+  int32_t* h_count = nullptr;
+  int32_t* d_count = nullptr;
+  int64_t pool_elems = 0, spool_elems = 0;
+  std::vector<Req> inflight;
+  bool busy = false;
+  bool failed = false;  // the launch failed: its requests complete as undecided
+  size_t bytes = 0;  // device bytes
+};
+
+struct Shape {
+  int m = 0, n = 0;
+  bool sense = false;
+  std::deque<Req> queue;
+  std::vector<std::unique_ptr<Arena>> arenas;
+};
+
+}  // namespace
+
+struct blpg_server {
+  blpg_ctx* parent = nullptr;
+  blpg_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::map<std::tuple<int, int, int>, std::unique_ptr<Shape>> shapes;
+  std::deque<blpg_server_result> done;
+  std::vector<double> lat;  // all completion latencies
+  int64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, batches = 0,
+          batched_lps = 0;
+  int64_t arena_bytes = 0, pending_bytes = 0;
+  std::string err;
+  std::mutex create_mu;  // serialises the creation of a new shape's arenas
+  std::atomic<bool> stop{false};
+  std::thread worker;
+};
+
+namespace {
+
+int64_t req_bytes(int m, int n, bool sense) {
+  return 8 * ((int64_t)m * n + m + n) + (sense ? m : 0) + 12 * (int64_t)(m + n) + 64;
+}
+
+void destroy_arena(Arena& a) {
+  if (a.B) blpg_batch_free(a.ctx, a.B);
+  if (a.d_count) (void)hipFree(a.d_count);
+  if (a.h_pool) (void)hipHostFree(a.h_pool);
+  if (a.h_sense) (void)hipHostFree(a.h_sense);
+  if (a.h_count) (void)hipHostFree(a.h_count);
+  if (a.ctx) blpg_destroy(a.ctx);
+  a = Arena{};
+}
+
+// An arena: a device batch of `cap` distinct slots of shape (m, n), uploaded once with
+// zero data (the pool layout is what matters), plus its pinned staging mirrors.
+int make_arena(blpg_server* S, Shape& sh, Arena& a) {
+  const int cap = S->cfg.max_batch;
+  blpg_options o;
+  blpg_default_options(&o);
+  o.precision = S->cfg.precision;
+  o.device = S->parent->opt.device;
+  o.use_graph = 1;
+  int rc = blpg_create(&a.ctx, &o);
+  if (rc) return rc;
+  const std::string tune = S->parent->tuner.export_text();
+  if (!tune.empty()) (void)blpg_tune_import(a.ctx, tune.c_str());
+  const int m = sh.m, n = sh.n;
+  std::vector<double> zeros((size_t)cap * ((size_t)m * n + m + n), 0.0);
+  std::vector<int8_t> szero(sh.sense ? (size_t)cap * m : 1, 0);
+  std::vector<blpg_lp> lps(cap);
+  for (int s = 0; s < cap; ++s) {
+    double* base = zeros.data() + (size_t)s * ((size_t)m * n + m + n);
+    lps[s].m = m;
+    lps[s].n = n;
+    lps[s].A = base;
+    lps[s].b = base + (size_t)m * n;
+    lps[s].c = base + (size_t)m * n + m;
+    lps[s].sense = sh.sense ? szero.data() + (size_t)s * m : nullptr;
+    lps[s].family = -1;
+    lps[s].warm_basis = nullptr;
+  }
+  rc = blpg_batch_upload(a.ctx, lps.data(), cap, &a.B);
+  if (rc) { S->err = blpg_last_error(a.ctx); return rc; }
+  a.pool_elems = a.B->pool_elems;
+  a.spool_elems = a.B->spool_elems;
+  if (hipMalloc(&a.d_count, sizeof(int32_t)) != hipSuccess ||
+      hipHostMalloc((void**)&a.h_count, sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&a.h_pool, sizeof(double) * a.pool_elems, hipHostMallocDefault) != hipSuccess ||
+      (sh.sense && hipHostMalloc((void**)&a.h_sense, std::max<int64_t>(a.spool_elems, 1),
+                                 hipHostMallocDefault) != hipSuccess))
+    return BLPG_E_NOMEM;
+  std::memset(a.h_pool, 0, sizeof(double) * a.pool_elems);
+  if (a.h_sense) std::memset(a.h_sense, 0, a.spool_elems);
+  a.B->d_count = a.d_count;
+  a.bytes = a.B->mem_bytes + a.B->res_bytes;
+  return BLPG_OK;
+}
+
+// Stage `reqs` into slots 0..k-1 of the arena and replay its graph (asynchronous).
+int launch(Arena& a, const std::vector<Req>& reqs) {
+  blpg_batch* B = a.B;
+  const int k = (int)reqs.size();
+  int64_t end = 0, send = 0;
+  for (int s = 0; s < k; ++s) {
+    const LpDesc& d = B->desc[s];
+    const Req& r = reqs[s];
+    const size_t mn = (size_t)d.m * d.n;
+    std::memcpy(a.h_pool + d.a, r.data.data(), sizeof(double) * mn);
+    std::memcpy(a.h_pool + d.b, r.data.data() + mn, sizeof(double) * d.m);
+    std::memcpy(a.h_pool + d.c, r.data.data() + mn + d.m, sizeof(double) * d.n);
+    end = std::max<int64_t>(end, d.c + d.n);
+    if (a.h_sense && d.s >= 0) {
+      std::memcpy(a.h_sense + d.s, r.sense.data(), d.m);
+      send = std::max<int64_t>(send, d.s + d.m);
+    }
+  }
+  *a.h_count = k;
+  hipStream_t st = a.ctx->stream;
+  if (hipMemcpyAsync(B->d_pool, a.h_pool, sizeof(double) * end, hipMemcpyHostToDevice, st) != hipSuccess ||
+      (send > 0 && hipMemcpyAsync(B->d_spool, a.h_sense, send, hipMemcpyHostToDevice, st) != hipSuccess) ||
+      hipMemcpyAsync(a.d_count, a.h_count, sizeof(int32_t), hipMemcpyHostToDevice, st) != hipSuccess)
+    return BLPG_E_HIP;
+  return blpg_batch_solve(a.ctx, B);
+}
+
+// Results of a finished arena (only the used slots are copied back).
+int finish(blpg_server* S, Arena& a) {
+  blpg_batch_results R;
+  const int64_t k = (int64_t)a.inflight.size();
+  const int rc = a.failed ? BLPG_E_HIP : batch_fetch_prefix(a.ctx, a.B, k, &R);
+  a.failed = false;
+  const auto now = Clock::now();
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (int64_t s = 0; s < k; ++s) {
+    Req& r = a.inflight[s];
+    blpg_server_result o{};
+    o.ticket = r.ticket;
+    o.status = rc ? BLPG_UNDECIDED : R.status[s];
+    o.pivots = rc ? 0 : R.pivots[s];
+    o.objective = rc ? 0.0 : R.objective[s];
+    o.latency_us = std::chrono::duration<double, std::micro>(now - r.t0).count();
+    if (!rc) {
+      if (r.x) std::memcpy(r.x, R.x + R.x_offset[s], sizeof(double) * r.n);
+      if (r.basis) std::memcpy(r.basis, R.basis + R.basis_offset[s], sizeof(int32_t) * r.m);
+    }
+    S->lat.push_back(o.latency_us);
+    S->done.push_back(o);
+    S->pending_bytes -= r.bytes;
+    ++S->completed;
+  }
+  a.inflight.clear();
+  a.busy = false;
+  S->cv_done.notify_all();
+  return rc;
+}
+
+void worker_loop(blpg_server* S) {
+  (void)hipSetDevice(S->parent->device);
+  const auto max_wait = std::chrono::microseconds(S->cfg.max_wait_us);
+  while (!S->stop.load()) {
+    bool progress = false;
+    // completions
+    std::vector<Arena*> ready;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      for (auto& kv : S->shapes)
+        for (auto& a : kv.second->arenas)
+          if (a->busy && hipStreamQuery(a->ctx->stream) == hipSuccess) ready.push_back(a.get());
+    }
+    for (Arena* a : ready) { finish(S, *a); progress = true; }
+    // launches
+    std::vector<std::pair<Arena*, std::vector<Req>>> todo;
+    std::vector<std::pair<Shape*, Arena*>> fresh;  // shapes that need a new arena
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      const auto now = Clock::now();
+      for (auto& kv : S->shapes) {
+        Shape& sh = *kv.second;
+        if (sh.queue.empty()) continue;
+        Arena* freea = nullptr;
+        bool any_busy = false;
+        for (auto& a : sh.arenas) {
+          if (a->busy) any_busy = true;
+          else if (!freea) freea = a.get();
+        }
+        const bool full = (int)sh.queue.size() >= S->cfg.max_batch;
+        const bool aged = now - sh.queue.front().t0 >= max_wait;
+        if (!(full || aged || !any_busy)) continue;
+        if (!freea) continue;
+        const int k = std::min<int>((int)sh.queue.size(), S->cfg.max_batch);
+        std::vector<Req> reqs;
+        reqs.reserve(k);
+        for (int i = 0; i < k; ++i) {
+          reqs.push_back(std::move(sh.queue.front()));
+          sh.queue.pop_front();
+        }
+        todo.push_back({freea, std::move(reqs)});
+        freea->busy = true;  // reserved
+      }
+    }
+    for (auto& t : todo) {
+      const int k = (int)t.second.size();
+      const int rc = launch(*t.first, t.second);
+      std::lock_guard<std::mutex> lk(S->mu);
+      t.first->inflight = std::move(t.second);  // busy stays set: finished by the poll above
+      if (rc) {
+        S->err = "launch failed";
+        t.first->failed = true;
+      } else {
+        ++S->batches;
+        S->batched_lps += k;
+      }
+      progress = true;
+    }
+    if (!progress) {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_work.wait_for(lk, std::chrono::microseconds(20));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int blpg_server_create(blpg_ctx* ctx, const blpg_server_config* cfg, blpg_server** out) {
+  if (!ctx || !out) return BLPG_E_INVALID;
+  *out = nullptr;
+  blpg_server* S = new blpg_server();
+  S->parent = ctx;
+  if (cfg) S->cfg = *cfg;
+  if (S->cfg.hbm_budget <= 0) S->cfg.hbm_budget = 64ll << 30;
+  if (S->cfg.max_batch <= 0) S->cfg.max_batch = 2048;
+  if (S->cfg.max_wait_us <= 0) S->cfg.max_wait_us = 200;
+  if (S->cfg.arenas <= 0) S->cfg.arenas = 2;
+  if (S->cfg.precision != BLPG_FP32) S->cfg.precision = BLPG_FP64;
+  S->worker = std::thread(worker_loop, S);
+  *out = S;
+  return BLPG_OK;
+}
+
+void blpg_server_destroy(blpg_server* S) {
+  if (!S) return;
+  S->stop.store(true);
+  S->cv_work.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  for (auto& kv : S->shapes)
+    for (auto& a : kv.second->arenas) {
+      if (a->ctx) (void)hipStreamSynchronize(a->ctx->stream);
+      destroy_arena(*a);
+    }
+  delete S;
+}
+
+}  // extern "C"
+
+namespace {
+
+int submit_at(blpg_server* S, const blpg_lp* lp, double* x, int32_t* basis, Clock::time_point t0,
+              int64_t* ticket) {
+  if (!S || !lp || !lp->A || !lp->b || !lp->c || lp->m < 1 || lp->n < 1 || lp->m > 4096 ||
+      lp->n > 4096)
+    return BLPG_E_INVALID;
+  const int m = lp->m, n = lp->n;
+  const size_t mn = (size_t)m * n;
+  for (size_t e = 0; e < mn; ++e)
+    if (!std::isfinite(lp->A[e])) return BLPG_E_INVALID;
+  for (int i = 0; i < m; ++i)
+    if (!std::isfinite(lp->b[i]) || (lp->sense && (lp->sense[i] < 0 || lp->sense[i] > 2)))
+      return BLPG_E_INVALID;
+  for (int j = 0; j < n; ++j)
+    if (!std::isfinite(lp->c[j])) return BLPG_E_INVALID;
+  const bool has_sense = lp->sense != nullptr;
+  const auto key = std::make_tuple(m, n, (int)has_sense);
+  Req r;
+  r.m = m;
+  r.n = n;
+  r.data.resize(mn + m + n);
+  std::memcpy(r.data.data(), lp->A, sizeof(double) * mn);
+  std::memcpy(r.data.data() + mn, lp->b, sizeof(double) * m);
+  std::memcpy(r.data.data() + mn + m, lp->c, sizeof(double) * n);
+  if (has_sense) r.sense.assign(lp->sense, lp->sense + m);
+  r.x = x;
+  r.basis = basis;
+  r.t0 = t0;
+  r.bytes = req_bytes(m, n, has_sense);
+  bool need;
+  {
+    std::lock_guard<std::mutex> lk0(S->mu);
+    need = S->shapes.find(key) == S->shapes.end();
+  }
+  std::unique_lock<std::mutex> ck(S->create_mu, std::defer_lock);
+  if (need) ck.lock();  // (never while holding mu: the creator takes mu after create_mu)
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->shapes.find(key);
+  if (it == S->shapes.end()) {
+    // a new shape: its arenas are created now (admitted against the budget too)
+    auto sh = std::make_unique<Shape>();
+    sh->m = m;
+    sh->n = n;
+    sh->sense = has_sense;
+    const int64_t arena_need = (int64_t)S->cfg.arenas * S->cfg.max_batch * r.bytes;
+    if (S->arena_bytes + S->pending_bytes + arena_need > S->cfg.hbm_budget) {
+      ++S->rejected;
+      return BLPG_E_BUDGET;
+    }
+    lk.unlock();  // arena creation allocates and uploads: do it without the lock
+    for (int i = 0; i < S->cfg.arenas; ++i) {
+      auto a = std::make_unique<Arena>();
+      const int rc = make_arena(S, *sh, *a);
+      if (rc) {
+        destroy_arena(*a);
+        for (auto& b : sh->arenas) destroy_arena(*b);
+        return rc;
+      }
+      sh->arenas.push_back(std::move(a));
+    }
+    lk.lock();
+    for (auto& a : sh->arenas) S->arena_bytes += (int64_t)a->bytes;
+    it = S->shapes.emplace(key, std::move(sh)).first;
+  }
+  if (S->arena_bytes + S->pending_bytes + r.bytes > S->cfg.hbm_budget) {
+    ++S->rejected;
+    return BLPG_E_BUDGET;
+  }
+  r.ticket = S->next_ticket++;
+  if (ticket) *ticket = r.ticket;
+  S->pending_bytes += r.bytes;
+  ++S->submitted;
+  it->second->queue.push_back(std::move(r));
+  lk.unlock();
+  S->cv_work.notify_one();
+  return BLPG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int blpg_server_submit(blpg_server* S, const blpg_lp* lp, double* x, int32_t* basis,
+                       int64_t* ticket) {
+  return submit_at(S, lp, x, basis, Clock::now(), ticket);
+}
+
+int blpg_server_poll(blpg_server* S, blpg_server_result* out, int32_t max, int32_t wait_us,
+                     int32_t* n) {
+  if (!S || (max > 0 && !out) || !n) return BLPG_E_INVALID;
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (S->done.empty() && wait_us > 0)
+    S->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !S->done.empty(); });
+  int32_t k = 0;
+  while (k < max && !S->done.empty()) {
+    out[k++] = S->done.front();
+    S->done.pop_front();
+  }
+  *n = k;
+  return BLPG_OK;
+}
+
+int blpg_server_stats_get(blpg_server* S, blpg_server_stats* st) {
+  if (!S || !st) return BLPG_E_INVALID;
+  std::lock_guard<std::mutex> lk(S->mu);
+  std::memset(st, 0, sizeof *st);
+  st->submitted = S->submitted;
+  st->completed = S->completed;
+  st->rejected = S->rejected;
+  st->batches = S->batches;
+  st->mean_batch = S->batches ? (double)S->batched_lps / (double)S->batches : 0.0;
+  st->resident_bytes = S->arena_bytes + S->pending_bytes;
+  if (!S->lat.empty()) {
+    std::vector<double> v = S->lat;
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double p) { return v[std::min(v.size() - 1, (size_t)(p * (double)(v.size() - 1) + 0.5))]; };
+    st->p50_us = pct(0.50);
+    st->p99_us = pct(0.99);
+    st->max_us = v.back();
+  }
+  return BLPG_OK;
+}
+
+int blpg_server_replay(blpg_server* S, const blpg_lp* lps, const double* arrival_us, int64_t count,
+                       blpg_server_result* out) {
+  if (!S || (count > 0 && (!lps || !arrival_us)) || count < 0) return BLPG_E_INVALID;
+  std::vector<int64_t> tickets(count, -1);
+  std::atomic<int> err{0};
+  const auto start = Clock::now();
+  std::thread pacer([&] {
+    for (int64_t k = 0; k < count && !err.load(); ++k) {
+      const auto due = start + std::chrono::nanoseconds((int64_t)(arrival_us[k] * 1000.0));
+      while (Clock::now() < due) std::this_thread::yield();
+      for (;;) {
+        int64_t t = -1;
+        // latency counts from the scheduled arrival
+        const int rc = submit_at(S, &lps[k], nullptr, nullptr, due, &t);
+        if (rc == BLPG_OK) {
+          tickets[k] = t;
+          break;
+        }
+        if (rc != BLPG_E_BUDGET) { err.store(rc); break; }
+        std::this_thread::yield();
+      }
+    }
+  });
+  std::map<int64_t, blpg_server_result> got;
+  std::vector<blpg_server_result> buf(1024);
+  while ((int64_t)got.size() < count && !err.load()) {
+    int32_t n = 0;
+    blpg_server_poll(S, buf.data(), (int32_t)buf.size(), 1000, &n);
+    for (int i = 0; i < n; ++i) got[buf[i].ticket] = buf[i];
+  }
+  pacer.join();
+  if (out)
+    for (int64_t k = 0; k < count; ++k) {
+      auto it = got.find(tickets[k]);
+      out[k] = it == got.end() ? blpg_server_result{tickets[k], BLPG_UNDECIDED, 0, 0.0, -1.0} : it->second;
+    }
+  return err.load();
+}
+
+}  // extern "C"
