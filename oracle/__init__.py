@@ -48,8 +48,8 @@ def lib() -> C.CDLL:
             getattr(L, nm).argtypes = [P(_LP), P(_Opts), P(_Res)]
         for nm in ("orc_solve_warm_d", "orc_solve_warm_f"):
             getattr(L, nm).argtypes = [P(_LP), P(_Opts), P(C.c_int32), P(_Res)]
-        L.orc_solve_tab_d.argtypes = [P(_LP), P(_Opts), P(_Res), P(C.c_double)]
-        L.orc_solve_tab_f.argtypes = [P(_LP), P(_Opts), P(_Res), P(C.c_float)]
+        L.orc_solve_tab_d.argtypes = [P(_LP), P(_Opts), P(_Res), P(C.c_double), P(C.c_int32)]
+        L.orc_solve_tab_f.argtypes = [P(_LP), P(_Opts), P(_Res), P(C.c_float), P(C.c_int32)]
         L.orc_ref_rule_solve.argtypes = [C.c_int32, C.c_int32, P(C.c_double), P(C.c_double),
                                          P(C.c_double), C.c_int32, P(C.c_double), P(C.c_int32)]
         L.orc_kkt_check.argtypes = [P(_LP), P(C.c_int32), P(C.c_double), P(C.c_double),
@@ -113,8 +113,10 @@ def solve(A, b, c, sense=None, precision: str = "fp64", tab: bool = False, warm=
         T = np.zeros((m + 2, n + 2), dtype=dt)
         fn = lib().orc_solve_tab_f if precision == "fp32" else lib().orc_solve_tab_d
         ptr = T.ctypes.data_as(C.POINTER(C.c_float if precision == "fp32" else C.c_double))
-        rc = fn(C.byref(L), C.byref(o), C.byref(r), ptr)
+        nb = np.zeros(n + 2, dtype=np.int32)
+        rc = fn(C.byref(L), C.byref(o), C.byref(r), ptr, nb.ctypes.data_as(C.POINTER(C.c_int32)))
         out["tab"] = T
+        out["nb"] = nb
     else:
         fn = lib().orc_solve_f if precision == "fp32" else lib().orc_solve_d
         rc = fn(C.byref(L), C.byref(o), C.byref(r))

@@ -66,9 +66,12 @@ typedef struct {
 /* returns 0, or -1 on invalid input */
 int orc_solve_d(const orc_lp *lp, const orc_opts *o, orc_res *res);
 int orc_solve_f(const orc_lp *lp, const orc_opts *o, orc_res *res);
-/* same, but also copies the final (m+2)x(n+2) tableau (row-major) into tab_out */
-int orc_solve_tab_d(const orc_lp *lp, const orc_opts *o, orc_res *res, double *tab_out);
-int orc_solve_tab_f(const orc_lp *lp, const orc_opts *o, orc_res *res, float *tab_out);
+/* same, but also copies the final (m+2)x(n+2) tableau (row-major) into tab_out and the
+ * variable id of every column (n+2; -1 for the rhs column) into nb_out */
+int orc_solve_tab_d(const orc_lp *lp, const orc_opts *o, orc_res *res, double *tab_out,
+                    int32_t *nb_out);
+int orc_solve_tab_f(const orc_lp *lp, const orc_opts *o, orc_res *res, float *tab_out,
+                    int32_t *nb_out);
 
 /* Warm start: `warm` (m variable ids, -1 = none) is a target basis whose structurals are
  * pivoted in first (crash: largest |T[i][s]| among rows whose basic variable is not a

@@ -28,19 +28,21 @@
 #undef SFX
 #undef FMA
 
-int orc_solve_d(const orc_lp *lp, const orc_opts *o, orc_res *res) { return solve_impl_d(lp, o, res, NULL, NULL); }
-int orc_solve_f(const orc_lp *lp, const orc_opts *o, orc_res *res) { return solve_impl_f(lp, o, res, NULL, NULL); }
-int orc_solve_tab_d(const orc_lp *lp, const orc_opts *o, orc_res *res, double *tab) {
-  return solve_impl_d(lp, o, res, tab, NULL);
+int orc_solve_d(const orc_lp *lp, const orc_opts *o, orc_res *res) { return solve_impl_d(lp, o, res, NULL, NULL, NULL); }
+int orc_solve_f(const orc_lp *lp, const orc_opts *o, orc_res *res) { return solve_impl_f(lp, o, res, NULL, NULL, NULL); }
+int orc_solve_tab_d(const orc_lp *lp, const orc_opts *o, orc_res *res, double *tab,
+                    int32_t *nb) {
+  return solve_impl_d(lp, o, res, tab, nb, NULL);
 }
-int orc_solve_tab_f(const orc_lp *lp, const orc_opts *o, orc_res *res, float *tab) {
-  return solve_impl_f(lp, o, res, tab, NULL);
+int orc_solve_tab_f(const orc_lp *lp, const orc_opts *o, orc_res *res, float *tab,
+                    int32_t *nb) {
+  return solve_impl_f(lp, o, res, tab, nb, NULL);
 }
 int orc_solve_warm_d(const orc_lp *lp, const orc_opts *o, const int32_t *warm, orc_res *res) {
-  return solve_impl_d(lp, o, res, NULL, warm);
+  return solve_impl_d(lp, o, res, NULL, NULL, warm);
 }
 int orc_solve_warm_f(const orc_lp *lp, const orc_opts *o, const int32_t *warm, orc_res *res) {
-  return solve_impl_f(lp, o, res, NULL, warm);
+  return solve_impl_f(lp, o, res, NULL, NULL, warm);
 }
 
 /* Batch driver: the CPU baseline of the benchmark.  Mirrors the reference's GLPK loop
@@ -65,7 +67,7 @@ int orc_solve_batch_d(int64_t count, const int32_t *m, const int32_t *n, const i
     lp.sense = (s_off && s_off[k] >= 0) ? spool + s_off[k] : NULL;
     orc_res r;
     memset(&r, 0, sizeof r);
-    if (solve_impl_d(&lp, o, &r, NULL, NULL) != 0) { bad = 1; continue; }
+    if (solve_impl_d(&lp, o, &r, NULL, NULL, NULL) != 0) { bad = 1; continue; }
     status[k] = r.status;
     pivots[k] = r.pivots;
     obj[k] = r.objective;

@@ -214,6 +214,8 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
     B->x_elems += L.n;
     d.basis = B->basis_elems;
     B->basis_elems += L.m;
+    d.t = B->tab_elems;
+    B->tab_elems += (int64_t)(L.m + 2) * (L.n + 2) + (L.n + 3) / 2;
     B->mmax = std::max(B->mmax, L.m);
     B->nmax = std::max(B->nmax, L.n);
   }
@@ -274,6 +276,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   const size_t o_f64 = off; off += al(sizeof(double) * n * 6);      // obj wfin chkobj kktp kktd amax
   const size_t o_x = off; off += al(sizeof(double) * B->x_elems);
   const size_t o_basis = off; off += al(sizeof(int32_t) * B->basis_elems);
+  const size_t o_tab = off; off += al(8 * (size_t)B->tab_elems);
   const size_t o_cnt = off; off += al(sizeof(int32_t) * 128);
   bool any_warm = false;
   for (int64_t k = 0; k < count; ++k) any_warm |= lps[k].warm_basis != nullptr;
@@ -304,7 +307,6 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->d_all = i32 + 6 * n;
   B->d_fail = i32 + 7 * n;
   int32_t* d_binidx = i32 + 8 * n;
-  B->d_big = i32 + 9 * n;
   double* f64 = (double*)(base + o_f64);
   B->out.obj = f64;
   B->out.wfin = f64 + n;
@@ -319,6 +321,8 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->basis_off.resize(n);
   for (int64_t k = 0; k < n; ++k) { B->x_off[k] = B->desc[k].x; B->basis_off[k] = B->desc[k].basis; }
   B->out.basis = (int32_t*)(base + o_basis);
+  B->d_tab = base + o_tab;
+  B->out.tab = nullptr;
   B->d_counters = (int32_t*)(base + o_cnt);
   if (any_warm) B->d_warm = (int32_t*)(base + o_warm);
 
@@ -478,7 +482,7 @@ struct SolvePlan {
   std::vector<BinPlan> plans, fplans;
   std::vector<Bin> fbins;
   std::vector<size_t> ws_off, fws_off, vws_off;
-  std::vector<int> vgrid, vgrid2, klds1, klds2;
+  std::vector<int> vgrid;
   std::vector<int64_t> vstride;
 };
 
@@ -489,7 +493,7 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
   const bool fp32 = precision == BLPG_FP32;
   P.plans.assign(nb, BinPlan{}); P.fplans.assign(nb, BinPlan{}); P.fbins.assign(nb, Bin{});
   P.ws_off.assign(nb, 0); P.fws_off.assign(nb, 0); P.vws_off.assign(nb, 0);
-  P.vgrid.assign(nb, 1); P.vgrid2.assign(nb, 1); P.klds1.assign(nb, 64); P.klds2.assign(nb, 64);
+  P.vgrid.assign(nb, 1);
   P.vstride.assign(nb, 0);
   std::vector<BinPlan>& plans = P.plans;
   std::vector<BinPlan>& fplans = P.fplans;
@@ -498,9 +502,6 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
   std::vector<size_t>& fws_off = P.fws_off;
   std::vector<size_t>& vws_off = P.vws_off;
   std::vector<int>& vgrid = P.vgrid;
-  std::vector<int>& vgrid2 = P.vgrid2;
-  std::vector<int>& klds1 = P.klds1;
-  std::vector<int>& klds2 = P.klds2;
   std::vector<int64_t>& vstride = P.vstride;
   size_t ws_total = 0, fws_total = 0, vws_total = 0;
   for (int i = 0; i < nb; ++i) {
@@ -511,17 +512,11 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
     ws_off[i] = ws_total;
     ws_total += plans[i].ws_bytes;
     if (do_verify) {
-      // first pass factors bases with k <= 32 in LDS (small footprint, many wavefronts per
-      // CU) and defers the rest to a second pass with k <= 64 in LDS (beyond: HBM slab)
       const int mc = b.rcap - 2, nc = b.ccap - 2;
-      const int64_t kcap = std::min(mc, nc);
-      klds1[i] = (int)std::min<int64_t>(32, kcap);
-      klds2[i] = (int)std::min<int64_t>(64, kcap);
-      vgrid[i] = persistent_grid(ctx, verify_max_blocks_per_cu(mc, nc, klds1[i]), cnt);
-      vgrid2[i] = persistent_grid(ctx, verify_max_blocks_per_cu(mc, nc, klds2[i]), cnt);
-      vstride[i] = (kcap * kcap + 63) & ~63ll;
+      vgrid[i] = persistent_grid(ctx, verify_max_blocks_per_cu(mc, nc), cnt);
+      vstride[i] = (int64_t)verify_ws_stride(mc, nc);
       vws_off[i] = vws_total;
-      vws_total += sizeof(double) * (size_t)vstride[i] * std::max(vgrid[i], vgrid2[i]);
+      vws_total += (size_t)vstride[i] * vgrid[i];
     }
     if (fp32) {
       // FP64 fallback kernel of this bin
@@ -583,14 +578,12 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
   std::vector<size_t>& fws_off = P.fws_off;
   std::vector<size_t>& vws_off = P.vws_off;
   std::vector<int>& vgrid = P.vgrid;
-  std::vector<int>& vgrid2 = P.vgrid2;
-  std::vector<int>& klds1 = P.klds1;
-  std::vector<int>& klds2 = P.klds2;
-  std::vector<int64_t>& vstride = P.vstride;
 
   if (timing) HIPCHK(ctx, hipEventRecord(B->ev[0], st));
   HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
   HIPCHK(ctx, launch_fill_i32(q, 0, 128, st));
+  // the solve kernels dump their final tableaux for the FP64 check
+  B->out.tab = do_verify ? B->d_tab : nullptr;
   // learned warm start: crash lists of the family LPs, before any solve kernel
   if (timing) HIPCHK(ctx, hipEventRecord(B->ev[kEvWsStart], st));
   if (B->model) {
@@ -645,31 +638,11 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
       V.mcap = b.rcap - 2;
       V.ncap = b.ccap - 2;
       V.grid = vgrid[i];
-      V.ws_stride = vstride[i];
-      V.ws = (double*)((char*)ctx->vws.p + vws_off[i]);
+      V.ws = (char*)ctx->vws.p + vws_off[i];
       V.queue = qb + 1;
       V.fail_list = B->d_fail + pos;
       V.fail_count = qb + 2;
-      V.klds = klds1[i];
-      const bool two_pass = klds2[i] > klds1[i];
-      if (two_pass) { V.big_list = B->d_big + pos; V.big_count = qb + 5; }
       HIPCHK(ctx, launch_verify(V, s));
-      if (two_pass) {
-        VerifyLaunch Vb = V;
-        Vb.idx = B->d_big + pos;
-        Vb.count = -1;
-        Vb.count_dev = qb + 5;
-        Vb.queue = qb + 6;
-        Vb.klds = klds2[i];
-        Vb.grid = vgrid2[i];
-        Vb.big_list = nullptr;
-        Vb.big_count = nullptr;
-        HIPCHK(ctx, launch_verify(Vb, s));
-      }
-      V.klds = klds2[i];
-      V.grid = vgrid2[i];
-      V.big_list = nullptr;
-      V.big_count = nullptr;
       if (timing) HIPCHK(ctx, hipEventRecord(B->ev[3 + 4 * i], s));
       if (fp32) {
         // FP64 re-solve of the LPs of this bin whose FP32 result failed; the count is

@@ -11,6 +11,8 @@ struct LpDesc {
   int32_t m, n;
   int64_t a, b, c, s;
   int64_t x, basis;
+  int64_t t;  // final-tableau dump: 8-byte units into Out::tab ((m+2)(n+2) entries, then
+              // the n+2 nonbasic variable ids as int32)
 };
 
 // Per-LP outputs of a solve (device arrays, indexed by LP id within the batch).
@@ -24,6 +26,8 @@ struct Out {
   double* x;        // pool
   int32_t* basis;   // pool
   double* amax;     // max |A_ij| of the LP's input (written by the solve kernels at set-up)
+  char* tab;        // final tableaux (row-major, solve precision) + nonbasic ids, for the
+                    // FP64 check; nullptr: not dumped
 };
 
 template <typename T>

@@ -68,7 +68,8 @@ struct blpg_batch {
   int64_t count = 0;
   int precision = 0;
   std::vector<blpg::LpDesc> desc;
-  int64_t pool_elems = 0, spool_elems = 0, x_elems = 0, basis_elems = 0;
+  int64_t pool_elems = 0, spool_elems = 0, x_elems = 0, basis_elems = 0, tab_elems = 0;
+  char* d_tab = nullptr;  // final-tableau dumps (read by the FP64 check)
   int mmax = 0, nmax = 0;
   void* mem = nullptr;
   size_t mem_bytes = 0;
@@ -85,7 +86,6 @@ struct blpg_batch {
   void* warm_mem = nullptr;       // warm lists allocated when a model was attached
   int32_t* d_all = nullptr;       // 0..count-1
   int32_t* d_fail = nullptr;      // LP ids failing the check
-  int32_t* d_big = nullptr;       // LP ids whose check needs the larger LDS block
   int32_t* d_count = nullptr;     // arena batches: number of LPs (slots 0..count-1) to solve
   int32_t* d_counters = nullptr;  // [0..nbins) queues, then verify/fallback queues and counts
   std::vector<blpg::Bin> bins;

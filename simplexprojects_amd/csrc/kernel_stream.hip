@@ -201,10 +201,12 @@ struct StreamTab {
     return bp;
   }
 
+  // A dead column never enters again and influences no other entry (an update of (i, j)
+  // reads only column s and row r), so it keeps its values: B^-1 columns of fixed slacks
+  // stay available to the FP64 check.
   __device__ __forceinline__ void kill_col(int j) {
     __syncthreads();
     if (threadIdx.x == 0) dead[j] = 1;
-    for (int i = threadIdx.x; i < R; i += NT) at(i, j) = T(0);
     __syncthreads();
   }
 
@@ -296,6 +298,13 @@ struct StreamTab {
     for (int i = 2 + tid; i < R; i += NT) {
       const int v = bs[i];
       if (v >= 0 && v < n) x[v] = (double)at(i, 0);
+    }
+    // final tableau and nonbasic ids for the FP64 check (B^-1 is in the slack columns)
+    if (out.tab) {
+      T* tp = (T*)(out.tab + 8 * d.t);
+      for (int e = tid; e < R * C; e += NT) tp[e] = tab[e];
+      int32_t* np = (int32_t*)(out.tab + 8 * (d.t + (int64_t)R * C));
+      for (int j = tid; j < C; j += NT) np[j] = nb[j];
     }
   }
 };

@@ -489,6 +489,21 @@ struct TileTab {
       const int var = bs[i];
       if (var >= 0 && var < n) x[var] = (double)rhsbuf[i];
     }
+    // final tableau and nonbasic ids for the FP64 check (B^-1 is in the slack columns)
+    if (out.tab) {
+      T* tp = (T*)(out.tab + 8 * d.t);
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int i = row_of(k);
+#pragma unroll
+        for (int l = 0; l < CPT; ++l) {
+          const int j = col_of(l);
+          if (i < R && j < C) tp[(int64_t)i * C + j] = v[k][l];
+        }
+      }
+      int32_t* np = (int32_t*)(out.tab + 8 * (d.t + (int64_t)R * C));
+      for (int j = tid; j < C; j += NT) np[j] = nb[j];
+    }
   }
 };
 

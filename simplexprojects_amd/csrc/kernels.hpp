@@ -70,24 +70,20 @@ struct VerifyLaunch {
   const int32_t* idx;  // LPs to check (nullptr: all)
   int32_t count;
   const int32_t* count_dev = nullptr;
-  Out out;             // status / basis / unb read from here
+  Out out;             // status / basis / unb / final tableaux read from here
   Check chk;
   double tol;          // relative KKT tolerance
-  int fp32_source;     // 1: results came from the FP32 tableau
-  double* ws;          // grid * ws_stride doubles (LU factors when they do not fit LDS)
-  int64_t ws_stride;
+  int fp32_source;     // 1: the tableaux are FP32 (else FP64)
+  void* ws;            // grid * verify_ws_stride bytes of scratch when it does not fit LDS
   int mcap, ncap;      // largest m, n
   int grid;
   int32_t* queue;
   int32_t* fail_list;  // LP ids that failed the check (appended)
   int32_t* fail_count;
-  int klds = 64;       // bases with k <= klds basic structurals factor M in LDS
-  int32_t* big_list = nullptr;   // if set, LPs with k > klds are appended here unchecked
-  int32_t* big_count = nullptr;
 };
 hipError_t launch_verify(const VerifyLaunch& L, hipStream_t st);
-int verify_max_blocks_per_cu(int mcap, int ncap, int klds);
-size_t verify_lds_bytes(int mcap, int ncap, int klds);
+int verify_max_blocks_per_cu(int mcap, int ncap);
+size_t verify_ws_stride(int mcap, int ncap);  // 0: scratch in LDS
 
 // small helpers (kernel_util.hip)
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);

@@ -1,0 +1,132 @@
+"""CPU tests of the FP64 refinement check's restatement (tests/verify_ref.py), pinned by
+HiGHS: a result the check accepts has HiGHS's objective; the check rejects the FP32 bases
+whose objective is wrong (near-singular bases of netlib BLEND / SC50A / SC105 / SC205 that
+FP32 pivoting ends on); a planted mistake in the tableau's inverse fails it.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import simplexprojects_amd as sa
+from tests.common import highs, load_highs, load_netlib, rel_err
+from tests.verify_ref import refine_check
+
+TOL = 1e-9
+STATUS = {oracle.OPTIMAL: "optimal", oracle.UNBOUNDED: "unbounded"}
+
+
+def check(A, b, c, sense, o):
+    return refine_check(A, b, c, sense, STATUS[o["status"]], o["basis"], o["tab"], o["nb"],
+                        o["unb_var"], float(np.abs(A).max()), TOL)
+
+
+def test_netlib_accepts_exactly_the_right_answers():
+    nl = load_netlib()
+    rejected = []
+    for name in sorted(nl):
+        A, b, c, mt = nl[name]
+        for prec in ("fp64", "fp32"):
+            o = oracle.solve(A, b, c, precision=prec, tab=True)
+            if o["status"] not in STATUS:
+                continue
+            r = check(A, b, c, None, o)
+            if prec == "fp64":
+                assert r["ok"], name
+            if r["ok"] and mt["highs_opt_min"] is not None:
+                assert rel_err(-r["objective"], mt["highs_opt_min"]) <= 1e-9, (name, prec)
+            if not r["ok"]:
+                rejected.append(name)
+    # FP32 ends on singular bases here (objective 0 instead of the optimum)
+    assert {"BLEND", "SC50A", "SC105", "SC205"} <= set(rejected)
+
+
+def mixed_sense_lps(count, m=30, n=26, seed=11):
+    """'<=', '=' and '>=' rows around a known feasible point, plus a bounding row."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        A = rng.uniform(-1, 3, (m, n))
+        x0 = rng.uniform(0, 2, n)
+        s = rng.choice([0, 1, 2], size=m, p=[0.5, 0.25, 0.25]).astype(np.int8)
+        s[-1] = 0
+        A[-1] = 1.0
+        u = rng.uniform(0.1, 1.0, m)
+        b = A @ x0 + np.where(s == 0, u, np.where(s == 2, -u, 0.0))
+        b[-1] = x0.sum() + 5.0
+        out.append((A, b, rng.uniform(-1, 2, n), s))
+    return out
+
+
+def check_point(A, b, sense, x):
+    ax = A @ x
+    scale = 1e-9 * (1 + np.abs(b).max() + np.abs(A).max() * np.abs(x).max())
+    assert np.all(ax[sense == 0] <= b[sense == 0] + scale)
+    assert np.all(ax[sense == 2] >= b[sense == 2] - scale)
+    assert np.all(np.abs(ax[sense == 1] - b[sense == 1]) <= scale)
+    assert np.all(x >= -scale)
+
+
+def test_mixed_senses_against_highs():
+    seen = 0
+    for A, b, c, s in mixed_sense_lps(12):
+        hs, hobj = highs(A, b, c, s)
+        for prec in ("fp64", "fp32"):
+            o = oracle.solve(A, b, c, s, precision=prec, tab=True)
+            assert o["status"] == hs
+            r = check(A, b, c, s, o)
+            if prec == "fp64":
+                assert r["ok"]
+            if r["ok"]:
+                seen += 1
+                assert rel_err(r["objective"], hobj) <= 1e-9
+                check_point(A, b, s, r["x"])
+    assert seen >= 12
+
+
+def test_generated_unbounded_and_optimal():
+    g = load_highs(2)
+    seen = {"optimal": 0, "unbounded": 0}
+    for grp in g["groups"][:2]:
+        lps = sa.gen_config(2, g["seed"], grp["m"], grp["n"], min(grp["count"], 32),
+                            shape_idx=grp["shape_idx"], first=grp["first"],
+                            batch_count=grp["batch_count"])
+        for k, lp in enumerate(lps):
+            o = oracle.solve(lp.A, lp.b, lp.c, lp.sense, tab=True)
+            if o["status"] not in STATUS:
+                continue
+            r = check(lp.A, lp.b, lp.c, lp.sense, o)
+            assert r["ok"], k
+            seen[STATUS[o["status"]]] += 1
+            if o["status"] == oracle.OPTIMAL:
+                assert rel_err(r["objective"], grp["objective"][k]) <= 1e-9
+                check_point(lp.A, lp.b, np.zeros(len(lp.b), np.int8), r["x"])
+    assert seen["optimal"] > 0 and seen["unbounded"] > 0
+
+
+def test_planted_mistakes_fail():
+    lps = sa.gen_config(1, 1, 28, 28, 6, shape_idx=0, batch_count=6)
+    for lp in lps:
+        o = oracle.solve(lp.A, lp.b, lp.c, lp.sense, tab=True)
+        assert o["status"] == oracle.OPTIMAL
+        assert check(lp.A, lp.b, lp.c, lp.sense, o)["ok"]
+        m, n = lp.A.shape
+        slack_cols = [j for j in range(1, n + 2) if n <= o["nb"][j] < n + m]
+        # a sign error in the column read as B^-1 e_p
+        bad = dict(o, tab=o["tab"].copy())
+        j = slack_cols[0]
+        bad["tab"][2:, j] *= -1
+        bad["tab"][0, j] *= -1
+        assert not check(lp.A, lp.b, lp.c, lp.sense, bad)["ok"]
+        # two slack columns swapped in the column map
+        if len(slack_cols) >= 2:
+            nb2 = o["nb"].copy()
+            a, b = slack_cols[:2]
+            nb2[a], nb2[b] = nb2[b], nb2[a]
+            assert not check(lp.A, lp.b, lp.c, lp.sense, dict(o, nb=nb2))["ok"]
+        # a wrong basis (one basic structural replaced by a nonbasic one)
+        bs = o["basis"].copy()
+        i = int(np.nonzero(bs < n)[0][0])
+        nonb = [v for v in range(n) if v not in set(bs.tolist())]
+        if nonb:
+            bs[i] = nonb[0]
+            assert not check(lp.A, lp.b, lp.c, lp.sense, dict(o, basis=bs))["ok"]
