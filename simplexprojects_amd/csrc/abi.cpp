@@ -272,7 +272,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   const size_t o_desc = off; off += al(sizeof(LpDesc) * n);
   const size_t o_pool = off; off += al(sizeof(double) * B->pool_elems);
   const size_t o_spool = off; off += al(B->spool_elems + 16);
-  const size_t o_i32 = off; off += al(sizeof(int32_t) * n * 10);    // status pivots p1 unb ok fb all fail idx big
+  const size_t o_i32 = off; off += al(sizeof(int32_t) * n * 10);    // status pivots p1 unb ok fb all fail idx defer
   const size_t o_f64 = off; off += al(sizeof(double) * n * 6);      // obj wfin chkobj kktp kktd amax
   const size_t o_x = off; off += al(sizeof(double) * B->x_elems);
   const size_t o_basis = off; off += al(sizeof(int32_t) * B->basis_elems);
@@ -306,6 +306,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->chk.fb = B->d_fb;
   B->d_all = i32 + 6 * n;
   B->d_fail = i32 + 7 * n;
+  B->d_defer = i32 + 9 * n;
   int32_t* d_binidx = i32 + 8 * n;
   double* f64 = (double*)(base + o_f64);
   B->out.obj = f64;
@@ -482,7 +483,7 @@ struct SolvePlan {
   std::vector<BinPlan> plans, fplans;
   std::vector<Bin> fbins;
   std::vector<size_t> ws_off, fws_off, vws_off;
-  std::vector<int> vgrid;
+  std::vector<int> vgrid, vgrid1;  // four-wavefront / one-wavefront check grids
   std::vector<int64_t> vstride;
 };
 
@@ -494,6 +495,7 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
   P.plans.assign(nb, BinPlan{}); P.fplans.assign(nb, BinPlan{}); P.fbins.assign(nb, Bin{});
   P.ws_off.assign(nb, 0); P.fws_off.assign(nb, 0); P.vws_off.assign(nb, 0);
   P.vgrid.assign(nb, 1);
+  P.vgrid1.assign(nb, 0);
   P.vstride.assign(nb, 0);
   std::vector<BinPlan>& plans = P.plans;
   std::vector<BinPlan>& fplans = P.fplans;
@@ -513,7 +515,9 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
     ws_total += plans[i].ws_bytes;
     if (do_verify) {
       const int mc = b.rcap - 2, nc = b.ccap - 2;
-      vgrid[i] = persistent_grid(ctx, verify_max_blocks_per_cu(mc, nc), cnt);
+      vgrid[i] = persistent_grid(ctx, verify_max_blocks_per_cu(false, mc, nc), cnt);
+      if (verify_use_wave(mc, nc, fp32 ? 4 : 8))
+        P.vgrid1[i] = persistent_grid(ctx, verify_max_blocks_per_cu(true, mc, nc), cnt);
       vstride[i] = (int64_t)verify_ws_stride(mc, nc);
       vws_off[i] = vws_total;
       vws_total += (size_t)vstride[i] * vgrid[i];
@@ -642,6 +646,20 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
       V.queue = qb + 1;
       V.fail_list = B->d_fail + pos;
       V.fail_count = qb + 2;
+      if (P.vgrid1[i] > 0) {
+        // one wavefront per LP first; the LPs whose blocks exceed its LDS go on to the
+        // four-wavefront kernel (count read on the device)
+        VerifyLaunch V1 = V;
+        V1.wave = true;
+        V1.grid = P.vgrid1[i];
+        V1.defer_list = B->d_defer + pos;
+        V1.defer_count = qb + 5;
+        HIPCHK(ctx, launch_verify(V1, s));
+        V.idx = B->d_defer + pos;
+        V.count = -1;
+        V.count_dev = qb + 5;
+        V.queue = qb + 6;
+      }
       HIPCHK(ctx, launch_verify(V, s));
       if (timing) HIPCHK(ctx, hipEventRecord(B->ev[3 + 4 * i], s));
       if (fp32) {
@@ -651,6 +669,7 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
                         qb + 3, (char*)ctx->ws2.p + fws_off[i], B->out.basis, s);
         if (rc) return rc;
         VerifyLaunch V2 = V;
+        V2.wave = false;
         V2.idx = B->d_fail + pos;
         V2.count = -1;
         V2.count_dev = qb + 2;

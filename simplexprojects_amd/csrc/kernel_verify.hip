@@ -1,11 +1,11 @@
 // kernel_verify.hip -- FP64 check of final bases by iterative refinement (no reference
 // counterpart: the reference returns only T[M-2][N-1], simplex.cu:380, and never checks it).
 //
-// One workgroup (256 threads) per LP, persistent over a work queue.  The solve kernel
-// dumps its final tableau (solve precision) and the nonbasic variable of every column.
-// With A', b' the LP with '>=' rows negated and B the basis matrix of [A' I], the tableau
-// holds B^-1 column by column: the column of a nonbasic slack s_p is B^-1 e_p, and a basic
-// slack's column of B^-1 is the unit vector of its row.  So no factorisation is needed:
+// The solve kernel dumps its final tableau (solve precision) and the nonbasic variable of
+// every column.  With A', b' the LP with '>=' rows negated and B the basis matrix of
+// [A' I], the tableau holds B^-1 column by column: the column of a nonbasic slack s_p is
+// B^-1 e_p, and a basic slack's column of B^-1 is the unit vector of its row.  So no
+// factorisation is needed:
 //   x_B <- T[.,0];  repeat: r = b' - B x_B (FP64, from A),  x_B += B~^-1 r
 //   y_P <- T[0, slack columns];  repeat: rho = c_B - B^T y,  y += B~^-T rho
 // where B~^-1 is the tableau's (rounded) inverse: each sweep shrinks the residual by about
@@ -24,6 +24,15 @@
 //             |A|_max |d|), and c_e - y^T a'_e > tol s_D.
 // FP32 results that fail (or FP32 INFEASIBLE whose phase-I objective is not clearly
 // negative, or iteration-limit / numeric statuses) are appended to the fallback list.
+//
+// Every sweep touches only the k basic structural columns of A' and the k nonbasic slack
+// columns of the tableau (k = number of basic structurals).  Both m x k blocks are staged
+// once into LDS (row-major, odd row stride: row sweeps and column sweeps are both free of
+// bank conflicts), so the sweeps run from LDS.  Two kernels:
+//   k_verify<128>  first pass, two wavefronts per LP and a small LDS budget (many LPs per
+//                  CU); an LP whose blocks do not fit is deferred to a list;
+//   k_verify<256>  four wavefronts per LP, a large LDS budget; an LP that still does not fit
+//                  runs its sweeps from global memory (rows batched per wavefront).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -35,21 +44,26 @@ namespace blpg {
 
 namespace {
 
-constexpr int kVT = 256;  // threads per LP
-constexpr int kVW = kVT / 64;
 constexpr int kMaxSweeps = 12;
+constexpr int kRB = 8;  // rows per wavefront pass of the global-memory row sweeps
 
-// per-LP scratch (LDS when it fits, else a slab per workgroup in HBM)
+// per-LP vectors (LDS when they fit, else a slab per workgroup in HBM) for NT threads
 struct VLayout {
-  size_t x, r, y, rho, d, bs, rowof, colp, S, Srow, Pn, Pcol, total;
-  __host__ __device__ VLayout(int mcap, int ncap) {
+  size_t x, r, y, rho, d, w, bp, ae, cS, part, bs, rowof, colp, S, Srow, Pn, Pcol, total;
+  __host__ __device__ VLayout(int mcap, int ncap, int nt) {
     auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
     size_t o = 0;
     x = o; o += al(8 * (size_t)mcap);      // basic values by row
     r = o; o += al(8 * (size_t)mcap);      // residual by constraint
     y = o; o += al(8 * (size_t)mcap);      // duals by constraint
     rho = o; o += al(8 * (size_t)mcap);    // dual residual by row
-    d = o; o += al(8 * (size_t)mcap);      // ray by row
+    d = rho;                               // ray by row (after the duals are done)
+    w = o; o += al(8 * (size_t)mcap);      // gathered vector of the current sweep
+    bp = o; o += al(8 * (size_t)mcap);     // b' (signed rhs)
+    ae = bp;                               // a'_e, the entering column (after x is done)
+    cS = o; o += al(8 * (size_t)mcap);     // c of the basic structurals (S order)
+    part = o;                              // column-sum partials
+    o += al(8 * (size_t)(mcap + 1 > 2 * nt ? mcap + 1 : 2 * nt));
     bs = o; o += al(4 * (size_t)mcap);     // basic variable of each row
     rowof = o; o += al(4 * (size_t)(mcap + ncap + 1));  // row of each basic variable, -1
     colp = o; o += al(4 * (size_t)mcap);   // tableau column of each nonbasic slack, -1
@@ -60,6 +74,11 @@ struct VLayout {
     total = o;
   }
 };
+
+// LDS bytes of the staged blocks: A'[:, S] (FP64) and T[2.., Pcol] (tableau precision)
+__host__ __device__ inline size_t stage_bytes(int m, int k, int tsz) {
+  return (size_t)m * (size_t)(k | 1) * (size_t)(8 + tsz);
+}
 
 __device__ __forceinline__ double wmax(double v) {
 #pragma unroll
@@ -72,50 +91,63 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
-// workgroup reductions through `red` (kVW doubles), uniform result
+// workgroup reductions through `red` (NT/64 doubles), uniform result
+template <int NT>
 __device__ __forceinline__ double bmax(double v, double* red) {
   v = wmax(v);
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = red[0];
+  if constexpr (NT == 64) {
+    return v;
+  } else {
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
 #pragma unroll
-  for (int w = 1; w < kVW; ++w) r = fmax(r, red[w]);
-  return r;
+    for (int w = 1; w < NT / 64; ++w) r = fmax(r, red[w]);
+    return r;
+  }
 }
+template <int NT>
 __device__ __forceinline__ double bsum(double v, double* red) {
   v = wsum(v);
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = red[0];
+  if constexpr (NT == 64) {
+    return v;
+  } else {
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
 #pragma unroll
-  for (int w = 1; w < kVW; ++w) r += red[w];
-  return r;
+    for (int w = 1; w < NT / 64; ++w) r += red[w];
+    return r;
+  }
 }
 
 // compacts {i < cnt : pred(i)} in ascending order into out (and key(i) into out2);
-// returns the count (uniform).  Ballot prefix sums over 256-wide chunks.
-template <class Pred, class Key>
+// returns the count (uniform).  Ballot prefix sums over NT-wide chunks.
+template <int NT, class Pred, class Key>
 __device__ int compact(int cnt, Pred pred, Key key, int* out, int* out2, int* red_i) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   int base = 0;
-  for (int c0 = 0; c0 < cnt; c0 += kVT) {
+  for (int c0 = 0; c0 < cnt; c0 += NT) {
     const int i = c0 + tid;
     const bool p = i < cnt && pred(i);
     const unsigned long long bal = __ballot(p);
     const int before = __popcll(bal & ((1ull << lane) - 1ull));
-    __syncthreads();
-    if (lane == 0) red_i[w] = __popcll(bal);
-    __syncthreads();
-    int off = base;
-    for (int q = 0; q < w; ++q) off += red_i[q];
+    int off = base, tot = 0;
+    if constexpr (NT == 64) {
+      tot = __popcll(bal);
+    } else {
+      __syncthreads();
+      if (lane == 0) red_i[w] = __popcll(bal);
+      __syncthreads();
+      for (int q = 0; q < w; ++q) off += red_i[q];
+      for (int q = 0; q < NT / 64; ++q) tot += red_i[q];
+    }
     if (p) {
       out[off + before] = i;
       out2[off + before] = key(i);
     }
-    int tot = 0;
-    for (int q = 0; q < kVW; ++q) tot += red_i[q];
     base += tot;
   }
   __syncthreads();
@@ -124,11 +156,14 @@ __device__ int compact(int cnt, Pred pred, Key key, int* out, int* out2, int* re
 
 }  // namespace
 
-template <typename TT>
-__device__ void verify_lp(int lp, const LpDesc& D, const double* __restrict__ pool,
+// Checks one LP with NT threads.  wsp: the per-LP vectors; stg: `stage_cap` bytes of LDS
+// for the staged blocks.  Returns false (and writes nothing) when the blocks do not fit
+// and `defer` is set or NT < 256: the caller hands the LP to the four-wavefront kernel.
+template <typename TT, int NT>
+__device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ pool,
                           const int8_t* __restrict__ spool, Out out, Check chk, double tol,
-                          int fp32_source, char* wsp, double* red, int* red_i,
-                          int32_t* fail_list, int32_t* fail_count) {
+                          int fp32_source, char* wsp, char* stg, int64_t stage_cap, bool defer,
+                          double* red, int* red_i, int32_t* fail_list, int32_t* fail_count) {
   const int tid = threadIdx.x;
   const int m = D.m, n = D.n, R = m + 2, C = n + 2;
   const double* A = pool + D.a;
@@ -141,12 +176,17 @@ __device__ void verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   const int32_t* nbv = (const int32_t*)(out.tab + 8 * (D.t + (int64_t)R * C));
   auto sg = [&](int i) { return (sense && sense[i] == 2) ? -1.0 : 1.0; };
   auto is_eq = [&](int i) { return sense && sense[i] == 1; };
-  const VLayout L(m, n);
+  const VLayout L(m, n, NT);
   double* x = (double*)(wsp + L.x);
   double* r = (double*)(wsp + L.r);
   double* y = (double*)(wsp + L.y);
   double* rho = (double*)(wsp + L.rho);
   double* dv = (double*)(wsp + L.d);
+  double* w = (double*)(wsp + L.w);
+  double* bp = (double*)(wsp + L.bp);
+  double* ae = (double*)(wsp + L.ae);
+  double* cS = (double*)(wsp + L.cS);
+  double* part = (double*)(wsp + L.part);
   int* bs = (int*)(wsp + L.bs);
   int* rowof = (int*)(wsp + L.rowof);
   int* colp = (int*)(wsp + L.colp);
@@ -155,227 +195,407 @@ __device__ void verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   int* Pn = (int*)(wsp + L.Pn);
   int* Pcol = (int*)(wsp + L.Pcol);
 
-  double bm = 0, cm = 0;
-  for (int i = tid; i < m; i += kVT) bm = fmax(bm, fabs(b[i]));
-  for (int j = tid; j < n; j += kVT) cm = fmax(cm, fabs(c[j]));
-  bm = bmax(bm, red);
-  cm = bmax(cm, red);
-  const double am = out.amax[lp];
-
-  const double sq_tol = sqrt(tol);
-  int ok = 2;
-  double rp = -1, rd = -1, zobj = out.obj[lp];
   const bool checkable = (status == kStatusOptimal || status == kStatusUnbounded) && out.tab;
-  int kx = -1;
-  if (!checkable) {
-    if (fp32_source)
-      ok = (status == kStatusInfeasible && out.wfin[lp] < -1e-3 * (1.0 + bm)) ? 1 : 0;
-  } else {
+  int k = 0, bad = 0;
+  if (checkable) {
     // ---- basis bookkeeping ----
-    for (int v = tid; v < n + m + 1; v += kVT) rowof[v] = -1;
-    for (int p = tid; p < m; p += kVT) colp[p] = -1;
+    for (int v = tid; v < n + m + 1; v += NT) rowof[v] = -1;
+    for (int p = tid; p < m; p += NT) colp[p] = -1;
     __syncthreads();
-    int bad = 0;
-    for (int i = tid; i < m; i += kVT) {
+    for (int i = tid; i < m; i += NT) {
       const int v = basis[i];
       bs[i] = v;
       if (v < 0 || v >= n + m) bad = 1;
       else rowof[v] = i;
     }
-    for (int j = 1 + tid; j < C; j += kVT) {
+    for (int j = 1 + tid; j < C; j += NT) {
       const int v = nbv[j];
       if (v >= n && v < n + m) colp[v - n] = j;
     }
-    bad = (int)bmax((double)bad, red);
+    bad = (int)bmax<NT>((double)bad, red);
+    k = compact<NT>(n, [&](int j) { return rowof[j] >= 0; }, [&](int j) { return rowof[j]; },
+                    S, Srow, red_i);
+    const int kq = compact<NT>(m, [&](int p) { return colp[p] >= 0; },
+                               [&](int p) { return colp[p]; }, Pn, Pcol, red_i);
+    if (k != kq) bad = 1;
+  }
+  const bool fits = checkable && !bad && stage_bytes(m, k, (int)sizeof(TT)) <= (size_t)stage_cap;
+  if (checkable && !bad && !fits && (defer || NT < 256)) return false;
+  // the first-pass kernel only ever runs staged (its global-memory sweeps are not built)
+  const bool staged = NT < 256 || fits;
+
+  double bm = 0, cm = 0;
+  for (int i = tid; i < m; i += NT) bm = fmax(bm, fabs(b[i]));
+  for (int j = tid; j < n; j += NT) cm = fmax(cm, fabs(c[j]));
+  bm = bmax<NT>(bm, red);
+  cm = bmax<NT>(cm, red);
+  const double am = out.amax[lp];
+  const double sq_tol = sqrt(tol);
+
+  int ok = 2;
+  double rp = -1, rd = -1, zobj = out.obj[lp];
+  int kx = -1;
+  if (!checkable) {
+    if (fp32_source)
+      ok = (status == kStatusInfeasible && out.wfin[lp] < -1e-3 * (1.0 + bm)) ? 1 : 0;
+  } else if (bad) {
+    ok = 0;
+  } else {
+    const int kpad = k | 1;
+    double* AB = (double*)stg;                         // m x kpad: A'[p][S[a]]
+    TT* TB = (TT*)(stg + (size_t)m * kpad * 8);        // m x kpad: T[2+i][Pcol[a]]
+    for (int p = tid; p < m; p += NT) bp[p] = sg(p) * b[p];
+    for (int a = tid; a < k; a += NT) cS[a] = c[S[a]];
+    if (staged && k > 0) {
+      // one pass of gathers; thread (tp, ta) copies rows tp, tp + per, ... of column ta,
+      // eight rows per step (all loads of a step in flight together)
+      const int per = k >= NT ? 1 : NT / k;
+      const int ta = k >= NT ? tid : tid % k, tp = k >= NT ? 0 : tid / k;
+      if (tp < per) {
+        for (int a = ta; a < k; a += NT) {
+          const int ca = S[a], cp = Pcol[a];
+          int p = tp;
+          for (; p + 7 * per < m; p += 8 * per) {
+            double av[8];
+            TT tv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              av[u] = A[(int64_t)(p + u * per) * n + ca];
+              tv[u] = T[(int64_t)(2 + p + u * per) * C + cp];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              AB[(p + u * per) * kpad + a] = sg(p + u * per) * av[u];
+              TB[(p + u * per) * kpad + a] = tv[u];
+            }
+          }
+          for (; p < m; p += per) {
+            AB[p * kpad + a] = sg(p) * A[(int64_t)p * n + ca];
+            TB[p * kpad + a] = T[(int64_t)(2 + p) * C + cp];
+          }
+        }
+      }
+    }
     __syncthreads();
-    const int k = compact(n, [&](int j) { return rowof[j] >= 0; }, [&](int j) { return rowof[j]; },
-                          S, Srow, red_i);
-    const int kp = compact(m, [&](int p) { return colp[p] >= 0; }, [&](int p) { return colp[p]; },
-                           Pn, Pcol, red_i);
-    if (bad || k != kp) {
-      ok = 0;
-    } else {
-      const int lane = tid & 63, wave = tid >> 6;
-      // r = rhs - B v for the basis matrix B of [A' I]: one wavefront per constraint row,
-      // lanes over the basic structurals (ascending, so a row's loads stay together).
-      // rhs is b' (ray == false) or the entering column a'_e; returns the wave's max |r|.
-      auto resid = [&](const double* vb, bool ray, int e) {
-        double rl = 0;
-        for (int p = wave; p < m; p += kVW) {
-          const double* Ap = A + (int64_t)p * n;
-          double s = 0;
-          for (int a = lane; a < k; a += 64) s = fma(Ap[S[a]], vb[Srow[a]], s);
-          s = wsum(s);
+    const int lane = tid & 63, wave = tid >> 6;
+
+    // r = rhs - B v for the basis matrix B of [A' I]; rhs is b' (ray == false) or the
+    // entering column a'_e.  Returns the thread's max |r| (reduced by the caller).
+    auto resid = [&](const double* vb, bool ray, int e) {
+      double rl = 0;
+      if (staged) {
+        // one thread per constraint row over the staged block
+        for (int a = tid; a < k; a += NT) w[a] = vb[Srow[a]];
+        __syncthreads();
+        for (int p = tid; p < m; p += NT) {
+          const double* ABp = AB + p * kpad;
+          double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+          int a = 0;
+          for (; a + 4 <= k; a += 4) {
+            s0 = fma(ABp[a], w[a], s0);
+            s1 = fma(ABp[a + 1], w[a + 1], s1);
+            s2 = fma(ABp[a + 2], w[a + 2], s2);
+            s3 = fma(ABp[a + 3], w[a + 3], s3);
+          }
+          for (; a < k; ++a) s0 = fma(ABp[a], w[a], s0);
           const int q = rowof[n + p];
-          const double rhs = !ray ? b[p] : (e < n ? Ap[e] : (e - n == p ? sg(p) : 0.0));
-          const double val = sg(p) * (rhs - s) - (q >= 0 ? vb[q] : 0.0);
-          if (lane == 0) r[p] = val;
+          const double rhs = ray ? ae[p] : bp[p];
+          const double val = rhs - ((s0 + s1) + (s2 + s3)) - (q >= 0 ? vb[q] : 0.0);
+          r[p] = val;
           rl = fmax(rl, fabs(val));
         }
-        return rl;
-      };
-      // vb += B~^-1 r: row i of B~^-1 is the tableau row's slack columns, plus 1 at the
-      // row's own basic slack; one wavefront per row, lanes over the nonbasic slacks.
-      auto correct = [&](double* vb) {
-        for (int i = wave; i < m; i += kVW) {
-          const TT* Ti = T + (int64_t)(2 + i) * C;
-          double s = 0;
-          for (int a = lane; a < k; a += 64) s = fma((double)Ti[Pcol[a]], r[Pn[a]], s);
-          s = wsum(s);
-          const int v = bs[i];
-          if (lane == 0) vb[i] += s + (v >= n ? r[v - n] : 0.0);
+      } else {
+        // from global memory: each wavefront takes kRB rows at a time (kRB independent
+        // loads in flight per lane), lanes over the basic structurals (ascending, so a
+        // row's loads stay within a few cache lines)
+        for (int p0 = wave * kRB; p0 < m; p0 += (NT / 64) * kRB) {
+          double s[kRB];
+#pragma unroll
+          for (int u = 0; u < kRB; ++u) s[u] = 0;
+          for (int a = lane; a < k; a += 64) {
+            const int col = S[a];
+            const double xv = vb[Srow[a]];
+#pragma unroll
+            for (int u = 0; u < kRB; ++u) {
+              const int p = min(p0 + u, m - 1);
+              s[u] = fma(A[(int64_t)p * n + col], xv, s[u]);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kRB; ++u) {
+            const double tot = wsum(s[u]);
+            const int p = p0 + u;
+            if (p < m) {
+              const int q = rowof[n + p];
+              const double rhs = ray ? ae[p] : bp[p];
+              const double val = rhs - sg(p) * tot - (q >= 0 ? vb[q] : 0.0);
+              if (lane == 0) r[p] = val;
+              rl = fmax(rl, fabs(val));
+            }
+          }
         }
-        __syncthreads();
-      };
-      auto vmax = [&](const double* vb) {
-        double l = 0;
-        for (int i = tid; i < m; i += kVT) l = fmax(l, fabs(vb[i]));
-        return bmax(l, red);
-      };
-      // refinement: stops at the rounding floor, after kMaxSweeps, or when a sweep no
-      // longer shrinks the residual (a basis too ill-conditioned for the tableau's inverse)
-      auto refine = [&](double* vb, bool ray, int e, double scale0, double& res, double& vm) {
-        double prev = DBL_MAX;
-        for (int sweep = 0;; ++sweep) {
-          res = bmax(resid(vb, ray, e), red);
-          vm = vmax(vb);
-          if (sweep == kMaxSweeps || res <= 1e-14 * (scale0 + am * vm) || res > 0.9 * prev) break;
-          prev = res;
-          correct(vb);
-        }
-      };
-
-      // ---- x_B ----
-      for (int i = tid; i < m; i += kVT) x[i] = (double)T[(int64_t)(2 + i) * C];
-      __syncthreads();
-      double xm = 0, rres = 0;
-      refine(x, false, -1, 1.0 + bm, rres, xm);
-      double pv = 0;
-      for (int i = tid; i < m; i += kVT) {
-        const int v = bs[i];
-        const double xv = x[i];
-        if (v < n) pv = fmax(pv, -xv);
-        else pv = fmax(pv, is_eq(v - n) ? fabs(xv) : -xv);
       }
-      pv = bmax(pv, red);
-      rp = fmax(pv, rres) / (1.0 + bm + am * xm);
-      const bool pconv = rres <= sq_tol * (1.0 + bm);
-
-      // ---- y: rho = c_B - B^T y (basic structurals in S order, lanes along A's rows),
-      //      y += B~^-T rho (nonbasic slack p: its tableau column against rho) ----
-      for (int p = tid; p < m; p += kVT) y[p] = colp[p] >= 0 ? (double)T[colp[p]] : 0.0;
+      return rl;
+    };
+    // vb += B~^-1 r: row i of B~^-1 is the tableau row's slack columns, plus 1 at the
+    // row's own basic slack
+    auto correct = [&](double* vb) {
+      if (staged) {
+        for (int a = tid; a < k; a += NT) w[a] = r[Pn[a]];
+        __syncthreads();
+        for (int i = tid; i < m; i += NT) {
+          const TT* TBi = TB + i * kpad;
+          double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+          int a = 0;
+          for (; a + 4 <= k; a += 4) {
+            s0 = fma((double)TBi[a], w[a], s0);
+            s1 = fma((double)TBi[a + 1], w[a + 1], s1);
+            s2 = fma((double)TBi[a + 2], w[a + 2], s2);
+            s3 = fma((double)TBi[a + 3], w[a + 3], s3);
+          }
+          for (; a < k; ++a) s0 = fma((double)TBi[a], w[a], s0);
+          const int v = bs[i];
+          vb[i] += ((s0 + s1) + (s2 + s3)) + (v >= n ? r[v - n] : 0.0);
+        }
+      } else {
+        for (int i0 = wave * kRB; i0 < m; i0 += (NT / 64) * kRB) {
+          double s[kRB];
+#pragma unroll
+          for (int u = 0; u < kRB; ++u) s[u] = 0;
+          for (int a = lane; a < k; a += 64) {
+            const int col = Pcol[a];
+            const double rv = r[Pn[a]];
+#pragma unroll
+            for (int u = 0; u < kRB; ++u) {
+              const int i = min(i0 + u, m - 1);
+              s[u] = fma((double)T[(int64_t)(2 + i) * C + col], rv, s[u]);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kRB; ++u) {
+            const double tot = wsum(s[u]);
+            const int i = i0 + u;
+            if (i < m && lane == 0) {
+              const int v = bs[i];
+              vb[i] += tot + (v >= n ? r[v - n] : 0.0);
+            }
+          }
+        }
+      }
       __syncthreads();
-      double ym = 0, rres_d = 0, prev = DBL_MAX;
+    };
+    // column sums over a staged block: emit(a, sum_p M[p][a] v[p]) for a < k.  Thread
+    // (tp, ta) sums rows tp, tp + per, ... of column ta (all NT threads busy when k < NT);
+    // the per partials of a column are then added in order.
+    auto colsum = [&](const auto* M, const double* v, auto&& emit) {
+      const int per = k >= NT ? 1 : NT / k;
+      const int ta = k >= NT ? tid : tid % k, tp = k >= NT ? 0 : tid / k;
+      if (k > 0 && tp < per) {
+        for (int a = ta; a < k; a += NT) {
+          double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+          int p = tp;
+          for (; p + 3 * per < m; p += 4 * per) {
+            s0 = fma((double)M[p * kpad + a], v[p], s0);
+            s1 = fma((double)M[(p + per) * kpad + a], v[p + per], s1);
+            s2 = fma((double)M[(p + 2 * per) * kpad + a], v[p + 2 * per], s2);
+            s3 = fma((double)M[(p + 3 * per) * kpad + a], v[p + 3 * per], s3);
+          }
+          for (; p < m; p += per) s0 = fma((double)M[p * kpad + a], v[p], s0);
+          part[tp * kpad + a] = (s0 + s1) + (s2 + s3);
+        }
+      }
+      __syncthreads();
+      for (int a = tid; a < k; a += NT) {
+        double sum = 0;
+        for (int q = 0; q < per; ++q) sum += part[q * kpad + a];
+        emit(a, sum);
+      }
+    };
+    auto vmax = [&](const double* vb) {
+      double l = 0;
+      for (int i = tid; i < m; i += NT) l = fmax(l, fabs(vb[i]));
+      return bmax<NT>(l, red);
+    };
+    // refinement: stops at the rounding floor, after kMaxSweeps, or when a sweep no
+    // longer shrinks the residual (a basis too ill-conditioned for the tableau's inverse)
+    auto refine = [&](double* vb, bool ray, int e, double scale0, double& res, double& vm) {
+      double prev = DBL_MAX;
       for (int sweep = 0;; ++sweep) {
-        double rl = 0;
-        for (int a = tid; a < k; a += kVT) {
-          const int v = S[a];
-          double s0 = 0, s1 = 0;
+        res = bmax<NT>(resid(vb, ray, e), red);
+        vm = vmax(vb);
+        if (sweep == kMaxSweeps || res <= 1e-14 * (scale0 + am * vm) || res > 0.9 * prev) break;
+        prev = res;
+        correct(vb);
+      }
+    };
+
+    // ---- x_B ----
+    for (int i = tid; i < m; i += NT) x[i] = (double)T[(int64_t)(2 + i) * C];
+    __syncthreads();
+    double xm = 0, rres = 0;
+    refine(x, false, -1, 1.0 + bm, rres, xm);
+    double pv = 0;
+    for (int i = tid; i < m; i += NT) {
+      const int v = bs[i];
+      const double xv = x[i];
+      if (v < n) pv = fmax(pv, -xv);
+      else pv = fmax(pv, is_eq(v - n) ? fabs(xv) : -xv);
+    }
+    pv = bmax<NT>(pv, red);
+    rp = fmax(pv, rres) / (1.0 + bm + am * xm);
+    const bool pconv = rres <= sq_tol * (1.0 + bm);
+
+    // ---- y: rho = c_B - B^T y (one thread per basic structural), y += B~^-T rho (one
+    //      thread per nonbasic slack: its tableau column against rho) ----
+    for (int p = tid; p < m; p += NT) y[p] = colp[p] >= 0 ? (double)T[colp[p]] : 0.0;
+    __syncthreads();
+    double ym = 0, rres_d = 0, prev = DBL_MAX;
+    for (int sweep = 0;; ++sweep) {
+      double rl = 0;
+      if (staged) {
+        colsum(AB, y, [&](int a, double sum) {
+          const double val = cS[a] - sum;
+          rho[Srow[a]] = val;
+          rl = fmax(rl, fabs(val));
+        });
+      } else {
+        for (int a = tid; a < k; a += NT) {
+          double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
           int p = 0;
-          for (; p + 2 <= m; p += 2) {
+          const int v = S[a];
+          for (; p + 4 <= m; p += 4) {
             s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
             s1 = fma(sg(p + 1) * A[(int64_t)(p + 1) * n + v], y[p + 1], s1);
+            s2 = fma(sg(p + 2) * A[(int64_t)(p + 2) * n + v], y[p + 2], s2);
+            s3 = fma(sg(p + 3) * A[(int64_t)(p + 3) * n + v], y[p + 3], s3);
           }
-          if (p < m) s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
-          const double val = c[v] - (s0 + s1);
+          for (; p < m; ++p) s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
+          const double val = cS[a] - ((s0 + s1) + (s2 + s3));
           rho[Srow[a]] = val;
           rl = fmax(rl, fabs(val));
         }
-        for (int p = tid; p < m; p += kVT) {
-          const int q = rowof[n + p];
-          if (q >= 0) {
-            rho[q] = -y[p];
-            rl = fmax(rl, fabs(y[p]));
-          }
+      }
+      for (int p = tid; p < m; p += NT) {
+        const int q = rowof[n + p];
+        if (q >= 0) {
+          rho[q] = -y[p];
+          rl = fmax(rl, fabs(y[p]));
         }
-        rres_d = bmax(rl, red);
-        ym = vmax(y);
-        if (sweep == kMaxSweeps || rres_d <= 1e-14 * (1.0 + cm + am * ym) || rres_d > 0.9 * prev) break;
-        prev = rres_d;
-        for (int a = tid; a < k; a += kVT) {
-          const int j = Pcol[a];
-          double s0 = 0, s1 = 0;
+      }
+      rres_d = bmax<NT>(rl, red);
+      ym = vmax(y);
+      if (sweep == kMaxSweeps || rres_d <= 1e-14 * (1.0 + cm + am * ym) || rres_d > 0.9 * prev) break;
+      prev = rres_d;
+      if (staged) {
+        colsum(TB, rho, [&](int a, double sum) { y[Pn[a]] += sum; });
+      } else {
+        for (int a = tid; a < k; a += NT) {
+          double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
           int i = 0;
-          for (; i + 2 <= m; i += 2) {
+          const int j = Pcol[a];
+          for (; i + 4 <= m; i += 4) {
             s0 = fma((double)T[(int64_t)(2 + i) * C + j], rho[i], s0);
             s1 = fma((double)T[(int64_t)(3 + i) * C + j], rho[i + 1], s1);
+            s2 = fma((double)T[(int64_t)(4 + i) * C + j], rho[i + 2], s2);
+            s3 = fma((double)T[(int64_t)(5 + i) * C + j], rho[i + 3], s3);
           }
-          if (i < m) s0 = fma((double)T[(int64_t)(2 + i) * C + j], rho[i], s0);
-          y[Pn[a]] += s0 + s1;
+          for (; i < m; ++i) s0 = fma((double)T[(int64_t)(2 + i) * C + j], rho[i], s0);
+          y[Pn[a]] += (s0 + s1) + (s2 + s3);
         }
-        for (int p = tid; p < m; p += kVT) {
-          const int q = rowof[n + p];
-          if (q >= 0) y[p] += rho[q];
-        }
-        __syncthreads();
       }
-      // dual feasibility: nonbasic structurals and nonbasic '<=' slacks
-      double dvv = 0;
-      for (int j = tid; j < n; j += kVT) {
-        if (rowof[j] >= 0) continue;
-        double s0 = 0, s1 = 0;
-        int p = 0;
-        for (; p + 2 <= m; p += 2) {
-          s0 = fma(sg(p) * A[(int64_t)p * n + j], y[p], s0);
-          s1 = fma(sg(p + 1) * A[(int64_t)(p + 1) * n + j], y[p + 1], s1);
-        }
-        if (p < m) s0 = fma(sg(p) * A[(int64_t)p * n + j], y[p], s0);
-        dvv = fmax(dvv, c[j] - (s0 + s1));
+      for (int p = tid; p < m; p += NT) {
+        const int q = rowof[n + p];
+        if (q >= 0) y[p] += rho[q];
       }
-      for (int p = tid; p < m; p += kVT)
-        if (colp[p] >= 0 && !is_eq(p)) dvv = fmax(dvv, -y[p]);
-      rd = fmax(bmax(dvv, red), rres_d) / (1.0 + cm + am * ym);
-      const bool dconv = rres_d <= sq_tol * (1.0 + cm);
+      __syncthreads();
+    }
+    // dual feasibility: nonbasic structurals (one pass over A, coalesced across j) and
+    // nonbasic '<=' slacks
+    for (int p = tid; p < m; p += NT) w[p] = sg(p) * y[p];
+    __syncthreads();
+    double dvv = 0;
+    for (int j0 = tid; j0 < n; j0 += 2 * NT) {
+      // columns j0 and j0 + NT together: 16 independent loads per step
+      const int j1 = min(j0 + NT, n - 1);
+      double s0 = 0, s1 = 0, t0 = 0, t1 = 0;
+      int p = 0;
+      for (; p + 8 <= m; p += 8) {
+        double av[8], bv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          av[u] = A[(int64_t)(p + u) * n + j0];
+          bv[u] = A[(int64_t)(p + u) * n + j1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          s0 = fma(av[u], w[p + u], s0);
+          s1 = fma(av[u + 1], w[p + u + 1], s1);
+          t0 = fma(bv[u], w[p + u], t0);
+          t1 = fma(bv[u + 1], w[p + u + 1], t1);
+        }
+      }
+      for (; p < m; ++p) {
+        s0 = fma(A[(int64_t)p * n + j0], w[p], s0);
+        t0 = fma(A[(int64_t)p * n + j1], w[p], t0);
+      }
+      if (rowof[j0] < 0) dvv = fmax(dvv, c[j0] - (s0 + s1));
+      if (j0 + NT < n && rowof[j1] < 0) dvv = fmax(dvv, c[j1] - (t0 + t1));
+    }
+    for (int p = tid; p < m; p += NT)
+      if (colp[p] >= 0 && !is_eq(p)) dvv = fmax(dvv, -y[p]);
+    rd = fmax(bmax<NT>(dvv, red), rres_d) / (1.0 + cm + am * ym);
+    const bool dconv = rres_d <= sq_tol * (1.0 + cm);
 
-      // objective of the FP64 basic solution
-      double zp = 0;
-      for (int a = tid; a < k; a += kVT) zp = fma(c[S[a]], x[Srow[a]], zp);
-      const double zsum = bsum(zp, red);
-      kx = k;
-      if (status == kStatusOptimal) {
-        zobj = zsum;
-        ok = (rp <= tol && rd <= tol && pconv && dconv) ? 1 : 0;
-      } else {
-        // ---- unbounded: ray of the entering variable e, d = B^-1 a'_e by refinement ----
-        const int e = out.unb[lp];
-        int je = -1;
-        for (int j = 1 + tid; j < C; j += kVT)
-          if (nbv[j] == e) je = j;
-        je = (int)bmax((double)je, red);
-        bool good = e >= 0 && e < n + m && rowof[e] < 0 && je > 0 && rp <= tol && pconv;
-        if (good) {
-          for (int i = tid; i < m; i += kVT) dv[i] = (double)T[(int64_t)(2 + i) * C + je];
-          __syncthreads();
-          double dmax = 0, rres2 = 0;
-          refine(dv, true, e, 1.0, rres2, dmax);
-          double viol = 0;
-          for (int i = tid; i < m; i += kVT) {
-            const int bv = bs[i];
-            viol = fmax(viol, (bv >= n && is_eq(bv - n)) ? fabs(dv[i]) : dv[i]);
-          }
-          viol = bmax(viol, red);
-          double part = 0;
-          if (e < n) {
-            for (int p = tid; p < m; p += kVT) part = fma(y[p], sg(p) * A[(int64_t)p * n + e], part);
-          } else {
-            for (int p = tid; p < m; p += kVT) if (p == e - n) part = y[p];
-          }
-          const double re = (e < n ? c[e] : 0.0) - bsum(part, red);
-          good = viol <= tol * (1.0 + dmax) && rres2 <= tol * (1.0 + am * dmax) &&
-                 re > tol * (1.0 + cm + am * ym) && dconv;
+    // objective of the FP64 basic solution
+    double zp = 0;
+    for (int a = tid; a < k; a += NT) zp = fma(c[S[a]], x[Srow[a]], zp);
+    const double zsum = bsum<NT>(zp, red);
+    kx = k;
+    if (status == kStatusOptimal) {
+      zobj = zsum;
+      ok = (rp <= tol && rd <= tol && pconv && dconv) ? 1 : 0;
+    } else {
+      // ---- unbounded: ray of the entering variable e, d = B^-1 a'_e by refinement ----
+      const int e = out.unb[lp];
+      int je = -1;
+      for (int j = 1 + tid; j < C; j += NT)
+        if (nbv[j] == e) je = j;
+      je = (int)bmax<NT>((double)je, red);
+      bool good = e >= 0 && e < n + m && rowof[e] < 0 && je > 0 && rp <= tol && pconv;
+      if (good) {
+        for (int i = tid; i < m; i += NT) dv[i] = (double)T[(int64_t)(2 + i) * C + je];
+        for (int p = tid; p < m; p += NT)
+          ae[p] = e < n ? sg(p) * A[(int64_t)p * n + e] : (e - n == p ? 1.0 : 0.0);
+        __syncthreads();
+        double dmax = 0, rres2 = 0;
+        refine(dv, true, e, 1.0, rres2, dmax);
+        double viol = 0;
+        for (int i = tid; i < m; i += NT) {
+          const int bv = bs[i];
+          viol = fmax(viol, (bv >= n && is_eq(bv - n)) ? fabs(dv[i]) : dv[i]);
         }
-        ok = good ? 1 : 0;
-        rd = -1;
+        viol = bmax<NT>(viol, red);
+        double part = 0;
+        if (e < n) {
+          for (int p = tid; p < m; p += NT) part = fma(y[p], sg(p) * A[(int64_t)p * n + e], part);
+        } else {
+          for (int p = tid; p < m; p += NT) if (p == e - n) part = y[p];
+        }
+        const double re = (e < n ? c[e] : 0.0) - bsum<NT>(part, red);
+        good = viol <= tol * (1.0 + dmax) && rres2 <= tol * (1.0 + am * dmax) &&
+               re > tol * (1.0 + cm + am * ym) && dconv;
       }
+      ok = good ? 1 : 0;
+      rd = -1;
     }
   }
   __syncthreads();
   // a verified result reports the FP64 basic solution: x_S from the refinement, 0 elsewhere
   if (ok == 1 && kx >= 0) {
     double* xo = out.x + D.x;
-    for (int j = tid; j < n; j += kVT) xo[j] = 0.0;
+    for (int j = tid; j < n; j += NT) xo[j] = 0.0;
     __syncthreads();
-    for (int a = tid; a < kx; a += kVT) xo[S[a]] = x[Srow[a]];
+    for (int a = tid; a < kx; a += NT) xo[S[a]] = x[Srow[a]];
   }
   if (tid == 0) {
     chk.ok[lp] = ok;
@@ -392,71 +612,125 @@ __device__ void verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   // (nearly) optimal; other failed results restart cold (crash list cleared).
   if (fp32_source && ok == 0 && fail_list && status != kStatusOptimal) {
     int32_t* bw = out.basis + D.basis;
-    for (int i = tid; i < m; i += kVT) bw[i] = -1;
+    for (int i = tid; i < m; i += NT) bw[i] = -1;
   }
+  return true;
 }
 
-__global__ __launch_bounds__(kVT) void k_verify(const LpDesc* __restrict__ desc,
-                                                const double* __restrict__ pool,
-                                                const int8_t* __restrict__ spool,
-                                                const int32_t* __restrict__ idx, int32_t count,
-                                                const int32_t* count_dev, Out out, Check chk,
-                                                double tol, int fp32_source, char* ws,
-                                                int64_t ws_stride, int lds_scratch,
-                                                int32_t* queue, int32_t* fail_list,
-                                                int32_t* fail_count) {
+// Persistent, LPs assigned round-robin.  Dynamic LDS: [vectors (if lds_scratch)][stage_cap
+// bytes].
+// NT < 256 defers LPs whose staged blocks exceed stage_cap to defer_list.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_verify(const LpDesc* __restrict__ desc,
+                                               const double* __restrict__ pool,
+                                               const int8_t* __restrict__ spool,
+                                               const int32_t* __restrict__ idx, int32_t count,
+                                               const int32_t* count_dev, Out out, Check chk,
+                                               double tol, int fp32_source, char* ws,
+                                               int64_t ws_stride, int lds_scratch,
+                                               int64_t vec_bytes, int64_t stage_cap,
+                                               int32_t* queue, int32_t* fail_list,
+                                               int32_t* fail_count, int32_t* defer_list,
+                                               int32_t* defer_count) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_q;
-  __shared__ double red[kVW];
-  __shared__ int red_i[kVW];
+  __shared__ double red[NT / 64];
+  __shared__ int red_i[NT / 64];
   char* wsp = lds_scratch ? (char*)smem : ws + (int64_t)blockIdx.x * ws_stride;
+  char* stg = (char*)smem + (lds_scratch ? vec_bytes : 0);
+  const bool defer = NT < 256 && defer_list != nullptr;
   if (count_dev) count = *count_dev;
-  for (;;) {
+  // static round-robin over the LPs: the per-LP work is short and even, and one shared
+  // queue counter would serialise tens of thousands of atomics
+  (void)queue;
+  (void)s_q;
+  for (int q = blockIdx.x; q < count; q += gridDim.x) {
     __syncthreads();
-    if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
-    __syncthreads();
-    const int q = uniform(s_q);
-    if (q >= count) break;
     const int lp = idx ? idx[q] : q;
     const LpDesc D = desc[lp];
     // the dump's precision: FP32 tableau, or FP64 for FP64 solves and FP64 fallbacks
-    if (fp32_source)
-      verify_lp<float>(lp, D, pool, spool, out, chk, tol, 1, wsp, red, red_i, fail_list, fail_count);
-    else
-      verify_lp<double>(lp, D, pool, spool, out, chk, tol, 0, wsp, red, red_i, fail_list, fail_count);
+    const bool done =
+        fp32_source
+            ? verify_lp<float, NT>(lp, D, pool, spool, out, chk, tol, 1, wsp, stg, stage_cap,
+                                   defer, red, red_i, fail_list, fail_count)
+            : verify_lp<double, NT>(lp, D, pool, spool, out, chk, tol, 0, wsp, stg, stage_cap,
+                                    defer, red, red_i, fail_list, fail_count);
+    if (!done && threadIdx.x == 0) defer_list[atomicAdd(defer_count, 1)] = lp;
   }
 }
 
-size_t verify_scratch_bytes(int mcap, int ncap) { return VLayout(mcap, ncap).total; }
+namespace {
+constexpr int kFirstNT = 128;                  // threads per LP of the first-pass kernel
+constexpr size_t kFirstLds = 32 * 1024 - 64;   // its per-LP LDS (5 LPs per CU)
+constexpr size_t kBlockLds = 78 * 1024;        // per-LP LDS of the 256-thread kernel (2 per CU)
+constexpr size_t kVecLdsMax = 48 * 1024;
 
-static bool verify_in_lds(int mcap, int ncap) { return verify_scratch_bytes(mcap, ncap) <= 48 * 1024; }
+size_t vec_bytes(int mcap, int ncap, int nt) { return VLayout(mcap, ncap, nt).total; }
+bool vec_in_lds(int mcap, int ncap, int nt) { return vec_bytes(mcap, ncap, nt) <= kVecLdsMax; }
 
-int verify_max_blocks_per_cu(int mcap, int ncap) {
+// (stage_cap, total dynamic LDS) of a launch
+void lds_plan(bool first, int mcap, int ncap, int64_t& stage_cap, size_t& lds) {
+  const int nt = first ? kFirstNT : 256;
+  const size_t vb = vec_bytes(mcap, ncap, nt);
+  if (!vec_in_lds(mcap, ncap, nt)) { stage_cap = 0; lds = 0; return; }
+  const size_t budget = first ? kFirstLds : kBlockLds;
+  stage_cap = budget > vb ? (int64_t)(budget - vb) : 0;
+  // never more than the largest LP of the bin can use (FP64 tableau, k = min(m, n))
+  const int64_t most = (int64_t)stage_bytes(mcap, mcap < ncap ? mcap : ncap, 8);
+  if (stage_cap > most) stage_cap = most;
+  lds = vb + (size_t)stage_cap;
+}
+const void* kernel_of(bool first) {
+  return first ? reinterpret_cast<const void*>(&k_verify<kFirstNT>)
+               : reinterpret_cast<const void*>(&k_verify<256>);
+}
+}  // namespace
+
+bool verify_use_wave(int mcap, int ncap, int tsz) {
+  int64_t sc;
+  size_t lds;
+  lds_plan(true, mcap, ncap, sc, lds);
+  // worth a first pass when LPs with a dozen basic structurals fit (LPs that do not are
+  // handed on after their bookkeeping)
+  return vec_in_lds(mcap, ncap, kFirstNT) && sc >= (int64_t)stage_bytes(mcap, 12, tsz);
+}
+
+int verify_max_blocks_per_cu(bool first, int mcap, int ncap) {
   int nb = 0;
-  const void* fn = reinterpret_cast<const void*>(&k_verify);
-  const size_t lds = verify_in_lds(mcap, ncap) ? verify_scratch_bytes(mcap, ncap) : 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kVT, lds) != hipSuccess) return 0;
+  int64_t sc;
+  size_t lds;
+  lds_plan(first, mcap, ncap, sc, lds);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_of(first), first ? kFirstNT : 256,
+                                                   lds) != hipSuccess)
+    return 0;
   return nb;
 }
 
-// HBM scratch per workgroup when the LP's vectors exceed the LDS share (0: LDS)
+// HBM vectors per workgroup of the 256-thread kernel when they exceed the LDS share
+// (0: LDS); the first-pass kernel always keeps them in LDS
 size_t verify_ws_stride(int mcap, int ncap) {
-  return verify_in_lds(mcap, ncap) ? 0 : (verify_scratch_bytes(mcap, ncap) + 255) & ~(size_t)255;
+  return vec_in_lds(mcap, ncap, 256) ? 0 : (vec_bytes(mcap, ncap, 256) + 255) & ~(size_t)255;
 }
 
 hipError_t launch_verify(const VerifyLaunch& V, hipStream_t st) {
-  const bool lds = verify_in_lds(V.mcap, V.ncap);
-  const size_t bytes = lds ? verify_scratch_bytes(V.mcap, V.ncap) : 0;
-  const int lds_scratch = lds ? 1 : 0;
+  const int nt = V.wave ? kFirstNT : 256;
+  if (V.wave && (!V.defer_list || !V.defer_count || !vec_in_lds(V.mcap, V.ncap, nt)))
+    return hipErrorInvalidValue;
+  int64_t stage_cap;
+  size_t lds;
+  lds_plan(V.wave, V.mcap, V.ncap, stage_cap, lds);
+  const int lds_scratch = vec_in_lds(V.mcap, V.ncap, nt) ? 1 : 0;
+  const int64_t vb = (int64_t)vec_bytes(V.mcap, V.ncap, nt);
   char* ws = (char*)V.ws;
-  const int64_t stride = (int64_t)verify_ws_stride(V.mcap, V.ncap);
-  const void* fn = reinterpret_cast<const void*>(&k_verify);
+  const int64_t stride = V.wave ? 0 : (int64_t)verify_ws_stride(V.mcap, V.ncap);
+  int32_t* dl = V.wave ? V.defer_list : nullptr;
+  int32_t* dc = V.wave ? V.defer_count : nullptr;
   void* args[] = {(void*)&V.desc, (void*)&V.pool, (void*)&V.spool, (void*)&V.idx,
                   (void*)&V.count, (void*)&V.count_dev, (void*)&V.out, (void*)&V.chk,
                   (void*)&V.tol, (void*)&V.fp32_source, (void*)&ws, (void*)&stride,
-                  (void*)&lds_scratch, (void*)&V.queue, (void*)&V.fail_list,
-                  (void*)&V.fail_count};
-  return hipLaunchKernel(fn, dim3(V.grid), dim3(kVT), args, bytes, st);
+                  (void*)&lds_scratch, (void*)&vb, (void*)&stage_cap, (void*)&V.queue,
+                  (void*)&V.fail_list, (void*)&V.fail_count, (void*)&dl, (void*)&dc};
+  return hipLaunchKernel(kernel_of(V.wave), dim3(V.grid), dim3(nt), args, lds, st);
 }
 
 }  // namespace blpg
