@@ -88,14 +88,29 @@ struct TileTab {
     }
   }
 
+  // Loads the condensed tableau (row 0: -c, row 1: 0, rows 2..: A and b with '>=' rows
+  // negated) and max |A_ij|.  Every thread loads each of its entries from a valid address
+  // (a dummy one where the entry is structurally 0), so all loads are unconditional and
+  // go out together instead of one predicated load at a time.
   __device__ __forceinline__ void init(const double* A, const double* b, const double* c, const int8_t* sense) {
+    double am = 0.0;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
       const int i = row_of(k);
+      const bool crow = i >= 2 && i < R;
+      const int r = crow ? i - 2 : 0;
+      const bool neg = crow && sense && sense[r] == 2;
+      const double* Ar = A + (int64_t)r * n;
 #pragma unroll
       for (int l = 0; l < CPT; ++l) {
         const int j = col_of(l);
-        v[k][l] = (i < R && j < C) ? init_entry<T>(i, j, m, n, A, b, c, sense) : T(0);
+        const bool acol = j >= 1 && j <= n;
+        const bool isA = crow && acol, isB = crow && j == 0, isC = i == 0 && acol;
+        const double* src = isA ? Ar + (j - 1) : isB ? b + r : isC ? c + (j - 1) : b;
+        const double x = *src;
+        const bool ng = isC || (neg && (isA || isB));
+        v[k][l] = (isA || isB || isC) ? T(ng ? -x : x) : T(0);
+        if (isA) am = fmax(am, fabs(x));
       }
     }
     dead = 0;
@@ -105,16 +120,6 @@ struct TileTab {
       if (j == 0 || j >= C - 1) dead |= 1u << l;
     }
     // max |A_ij| in FP64 (the check scales its tolerances by it)
-    double am = 0.0;
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      const int i = row_of(k);
-#pragma unroll
-      for (int l = 0; l < CPT; ++l) {
-        const int j = col_of(l);
-        if (i >= 2 && i < R && j >= 1 && j <= n) am = fmax(am, fabs(A[(int64_t)(i - 2) * n + (j - 1)]));
-      }
-    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) am = fmax(am, __shfl_xor(am, off));
     if (lane == 0) s_am[tid >> 6] = am;
