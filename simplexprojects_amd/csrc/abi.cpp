@@ -272,7 +272,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   const size_t o_desc = off; off += al(sizeof(LpDesc) * n);
   const size_t o_pool = off; off += al(sizeof(double) * B->pool_elems);
   const size_t o_spool = off; off += al(B->spool_elems + 16);
-  const size_t o_i32 = off; off += al(sizeof(int32_t) * n * 10);    // status pivots p1 unb ok fb all fail idx defer
+  const size_t o_i32 = off; off += al(sizeof(int32_t) * n * 11);    // status pivots p1 unb ok fb all fail idx defer defer2
   const size_t o_f64 = off; off += al(sizeof(double) * n * 6);      // obj wfin chkobj kktp kktd amax
   const size_t o_x = off; off += al(sizeof(double) * B->x_elems);
   const size_t o_basis = off; off += al(sizeof(int32_t) * B->basis_elems);
@@ -307,6 +307,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->d_all = i32 + 6 * n;
   B->d_fail = i32 + 7 * n;
   B->d_defer = i32 + 9 * n;
+  B->d_defer2 = i32 + 10 * n;
   int32_t* d_binidx = i32 + 8 * n;
   double* f64 = (double*)(base + o_f64);
   B->out.obj = f64;
@@ -483,7 +484,7 @@ struct SolvePlan {
   std::vector<BinPlan> plans, fplans;
   std::vector<Bin> fbins;
   std::vector<size_t> ws_off, fws_off, vws_off;
-  std::vector<int> vgrid, vgrid1;  // four-wavefront / one-wavefront check grids
+  std::vector<std::vector<int>> vtiers;  // per bin: (tier, grid) pairs of the check, in order
   std::vector<int64_t> vstride;
 };
 
@@ -494,8 +495,7 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
   const bool fp32 = precision == BLPG_FP32;
   P.plans.assign(nb, BinPlan{}); P.fplans.assign(nb, BinPlan{}); P.fbins.assign(nb, Bin{});
   P.ws_off.assign(nb, 0); P.fws_off.assign(nb, 0); P.vws_off.assign(nb, 0);
-  P.vgrid.assign(nb, 1);
-  P.vgrid1.assign(nb, 0);
+  P.vtiers.assign(nb, {});
   P.vstride.assign(nb, 0);
   std::vector<BinPlan>& plans = P.plans;
   std::vector<BinPlan>& fplans = P.fplans;
@@ -503,7 +503,6 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
   std::vector<size_t>& ws_off = P.ws_off;
   std::vector<size_t>& fws_off = P.fws_off;
   std::vector<size_t>& vws_off = P.vws_off;
-  std::vector<int>& vgrid = P.vgrid;
   std::vector<int64_t>& vstride = P.vstride;
   size_t ws_total = 0, fws_total = 0, vws_total = 0;
   for (int i = 0; i < nb; ++i) {
@@ -515,12 +514,18 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
     ws_total += plans[i].ws_bytes;
     if (do_verify) {
       const int mc = b.rcap - 2, nc = b.ccap - 2;
-      vgrid[i] = persistent_grid(ctx, verify_max_blocks_per_cu(false, mc, nc), cnt);
-      if (verify_use_wave(mc, nc, fp32 ? 4 : 8))
-        P.vgrid1[i] = persistent_grid(ctx, verify_max_blocks_per_cu(true, mc, nc), cnt);
+      // tiers of growing LDS per LP; the last one used takes whatever the others defer
+      const int tsz = fp32 ? 4 : 8;
+      std::vector<int>& tv = P.vtiers[i];
+      for (int tier = 0; tier < 3; ++tier) {
+        if (!verify_tier_used(tier, mc, nc, tsz)) continue;
+        tv.push_back(tier);
+        tv.push_back(persistent_grid(ctx, verify_max_blocks_per_cu(tier, mc, nc), cnt));
+        if (tier < 2 && verify_tier_covers(tier, mc, nc, tsz)) break;
+      }
       vstride[i] = (int64_t)verify_ws_stride(mc, nc);
       vws_off[i] = vws_total;
-      vws_total += (size_t)vstride[i] * vgrid[i];
+      if (tv[tv.size() - 2] == 2) vws_total += (size_t)vstride[i] * tv.back();
     }
     if (fp32) {
       // FP64 fallback kernel of this bin
@@ -581,7 +586,6 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
   std::vector<size_t>& ws_off = P.ws_off;
   std::vector<size_t>& fws_off = P.fws_off;
   std::vector<size_t>& vws_off = P.vws_off;
-  std::vector<int>& vgrid = P.vgrid;
 
   if (timing) HIPCHK(ctx, hipEventRecord(B->ev[0], st));
   HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
@@ -641,26 +645,29 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
       V.fp32_source = fp32;
       V.mcap = b.rcap - 2;
       V.ncap = b.ccap - 2;
-      V.grid = vgrid[i];
       V.ws = (char*)ctx->vws.p + vws_off[i];
-      V.queue = qb + 1;
       V.fail_list = B->d_fail + pos;
       V.fail_count = qb + 2;
-      if (P.vgrid1[i] > 0) {
-        // one wavefront per LP first; the LPs whose blocks exceed its LDS go on to the
-        // four-wavefront kernel (count read on the device)
-        VerifyLaunch V1 = V;
-        V1.wave = true;
-        V1.grid = P.vgrid1[i];
-        V1.defer_list = B->d_defer + pos;
-        V1.defer_count = qb + 5;
-        HIPCHK(ctx, launch_verify(V1, s));
-        V.idx = B->d_defer + pos;
-        V.count = -1;
-        V.count_dev = qb + 5;
-        V.queue = qb + 6;
+      // tiers in sequence: the first takes the bin's LPs, each later one the LPs the
+      // previous deferred (lists alternate, counts read on the device)
+      const std::vector<int>& tv = P.vtiers[i];
+      const int ntier = (int)tv.size() / 2;
+      for (int u = 0; u < ntier; ++u) {
+        VerifyLaunch T = V;
+        T.tier = tv[2 * u];
+        T.grid = tv[2 * u + 1];
+        T.queue = qb + 1;
+        if (u > 0) {
+          T.idx = ((u - 1) % 2 == 0 ? B->d_defer : B->d_defer2) + pos;
+          T.count = -1;
+          T.count_dev = qb + ((u - 1) % 2 == 0 ? 5 : 6);
+        }
+        if (u + 1 < ntier) {
+          T.defer_list = (u % 2 == 0 ? B->d_defer : B->d_defer2) + pos;
+          T.defer_count = qb + (u % 2 == 0 ? 5 : 6);
+        }
+        HIPCHK(ctx, launch_verify(T, s));
       }
-      HIPCHK(ctx, launch_verify(V, s));
       if (timing) HIPCHK(ctx, hipEventRecord(B->ev[3 + 4 * i], s));
       if (fp32) {
         // FP64 re-solve of the LPs of this bin whose FP32 result failed; the count is
@@ -669,7 +676,9 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
                         qb + 3, (char*)ctx->ws2.p + fws_off[i], B->out.basis, s);
         if (rc) return rc;
         VerifyLaunch V2 = V;
-        V2.wave = false;
+        V2.tier = 2;
+        // (the same grid as a tier-2 pass over the bin, which sized the HBM vectors)
+        V2.grid = persistent_grid(ctx, verify_max_blocks_per_cu(2, V.mcap, V.ncap), cnt);
         V2.idx = B->d_fail + pos;
         V2.count = -1;
         V2.count_dev = qb + 2;

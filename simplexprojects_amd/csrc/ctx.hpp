@@ -86,7 +86,8 @@ struct blpg_batch {
   void* warm_mem = nullptr;       // warm lists allocated when a model was attached
   int32_t* d_all = nullptr;       // 0..count-1
   int32_t* d_fail = nullptr;      // LP ids failing the check
-  int32_t* d_defer = nullptr;     // LP ids the one-wavefront check hands on
+  int32_t* d_defer = nullptr;     // LP ids a check tier hands on (two lists, alternating)
+  int32_t* d_defer2 = nullptr;
   int32_t* d_count = nullptr;     // arena batches: number of LPs (slots 0..count-1) to solve
   int32_t* d_counters = nullptr;  // [0..nbins) queues, then verify/fallback queues and counts
   std::vector<blpg::Bin> bins;

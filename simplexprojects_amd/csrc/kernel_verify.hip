@@ -28,11 +28,11 @@
 // Every sweep touches only the k basic structural columns of A' and the k nonbasic slack
 // columns of the tableau (k = number of basic structurals).  Both m x k blocks are staged
 // once into LDS (row-major, odd row stride: row sweeps and column sweeps are both free of
-// bank conflicts), so the sweeps run from LDS.  Two kernels:
-//   k_verify<128>  first pass, two wavefronts per LP and a small LDS budget (many LPs per
-//                  CU); an LP whose blocks do not fit is deferred to a list;
-//   k_verify<256>  four wavefronts per LP, a large LDS budget; an LP that still does not fit
-//                  runs its sweeps from global memory (rows batched per wavefront).
+// bank conflicts), so the sweeps run from LDS.  Up to three tiers run in sequence on a
+// bin, each with a larger LDS budget per LP (fewer LPs per CU) and each taking the LPs the
+// previous one deferred: k_verify<128> with 32 KB, k_verify<256> with 78 KB, and
+// k_verify<256> with 150 KB, where an LP that still does not fit runs its sweeps from
+// global memory (rows batched per wavefront).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -46,6 +46,8 @@ namespace {
 
 constexpr int kMaxSweeps = 12;
 constexpr int kRB = 8;  // rows per wavefront pass of the global-memory row sweeps
+
+__host__ __device__ inline int part_cap(int mcap, int nt) { return mcap + 1 > nt ? mcap + 1 : nt; }
 
 // per-LP vectors (LDS when they fit, else a slab per workgroup in HBM) for NT threads
 struct VLayout {
@@ -62,8 +64,8 @@ struct VLayout {
     bp = o; o += al(8 * (size_t)mcap);     // b' (signed rhs)
     ae = bp;                               // a'_e, the entering column (after x is done)
     cS = o; o += al(8 * (size_t)mcap);     // c of the basic structurals (S order)
-    part = o;                              // column-sum partials
-    o += al(8 * (size_t)(mcap + 1 > 2 * nt ? mcap + 1 : 2 * nt));
+    part = o;                              // column-sum partials (part_cap doubles)
+    o += al(8 * (size_t)part_cap(mcap, nt));
     bs = o; o += al(4 * (size_t)mcap);     // basic variable of each row
     rowof = o; o += al(4 * (size_t)(mcap + ncap + 1));  // row of each basic variable, -1
     colp = o; o += al(4 * (size_t)mcap);   // tableau column of each nonbasic slack, -1
@@ -235,6 +237,7 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   int ok = 2;
   double rp = -1, rd = -1, zobj = out.obj[lp];
   int kx = -1;
+  bool keep_basis = status == kStatusOptimal;  // warm-start a fallback from this basis
   if (!checkable) {
     if (fp32_source)
       ok = (status == kStatusInfeasible && out.wfin[lp] < -1e-3 * (1.0 + bm)) ? 1 : 0;
@@ -388,7 +391,8 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
     // (tp, ta) sums rows tp, tp + per, ... of column ta (all NT threads busy when k < NT);
     // the per partials of a column are then added in order.
     auto colsum = [&](const auto* M, const double* v, auto&& emit) {
-      const int per = k >= NT ? 1 : NT / k;
+      const int pc = part_cap(m, NT) / kpad;
+      const int per = k >= NT ? 1 : (NT / k < pc ? NT / k : pc);
       const int ta = k >= NT ? tid : tid % k, tp = k >= NT ? 0 : tid / k;
       if (k > 0 && tp < per) {
         for (int a = ta; a < k; a += NT) {
@@ -444,6 +448,7 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
     pv = bmax<NT>(pv, red);
     rp = fmax(pv, rres) / (1.0 + bm + am * xm);
     const bool pconv = rres <= sq_tol * (1.0 + bm);
+    keep_basis = keep_basis || pconv;
 
     // ---- y: rho = c_B - B^T y (one thread per basic structural), y += B~^-T rho (one
     //      thread per nonbasic slack: its tableau column against rho) ----
@@ -608,9 +613,10 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
       chk.fb[lp] = 1;
     }
   }
-  // The fallback is warm-started from the FP32 basis only when that basis was
-  // (nearly) optimal; other failed results restart cold (crash list cleared).
-  if (fp32_source && ok == 0 && fail_list && status != kStatusOptimal) {
+  // The fallback is warm-started from the FP32 basis when that basis ended phase II as
+  // optimal, or as unbounded with a basic solution that refined (only the ray failed);
+  // other failed results restart cold (crash list cleared).
+  if (fp32_source && ok == 0 && fail_list && !keep_basis) {
     int32_t* bw = out.basis + D.basis;
     for (int i = tid; i < m; i += NT) bw[i] = -1;
   }
@@ -618,8 +624,8 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
 }
 
 // Persistent, LPs assigned round-robin.  Dynamic LDS: [vectors (if lds_scratch)][stage_cap
-// bytes].
-// NT < 256 defers LPs whose staged blocks exceed stage_cap to defer_list.
+// bytes].  With a defer_list, LPs whose staged blocks exceed stage_cap go to it (the next
+// tier takes them); NT < 256 always defers them.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_verify(const LpDesc* __restrict__ desc,
                                                const double* __restrict__ pool,
@@ -638,7 +644,7 @@ __global__ __launch_bounds__(NT) void k_verify(const LpDesc* __restrict__ desc,
   __shared__ int red_i[NT / 64];
   char* wsp = lds_scratch ? (char*)smem : ws + (int64_t)blockIdx.x * ws_stride;
   char* stg = (char*)smem + (lds_scratch ? vec_bytes : 0);
-  const bool defer = NT < 256 && defer_list != nullptr;
+  const bool defer = defer_list != nullptr;
   if (count_dev) count = *count_dev;
   // static round-robin over the LPs: the per-LP work is short and even, and one shared
   // queue counter would serialise tens of thousands of atomics
@@ -655,82 +661,101 @@ __global__ __launch_bounds__(NT) void k_verify(const LpDesc* __restrict__ desc,
                                    defer, red, red_i, fail_list, fail_count)
             : verify_lp<double, NT>(lp, D, pool, spool, out, chk, tol, 0, wsp, stg, stage_cap,
                                     defer, red, red_i, fail_list, fail_count);
-    if (!done && threadIdx.x == 0) defer_list[atomicAdd(defer_count, 1)] = lp;
+    if (!done && threadIdx.x == 0 && defer_list) defer_list[atomicAdd(defer_count, 1)] = lp;
   }
 }
 
 namespace {
-constexpr int kFirstNT = 128;                  // threads per LP of the first-pass kernel
-constexpr size_t kFirstLds = 32 * 1024 - 64;   // its per-LP LDS (5 LPs per CU)
-constexpr size_t kBlockLds = 78 * 1024;        // per-LP LDS of the 256-thread kernel (2 per CU)
+// Tiers of the check, run in sequence on a bin; each takes the LPs the previous one
+// deferred.  0: 128 threads and 32 KB per LP (5 LPs per CU), for LPs with few basic
+// structurals; 1: 256 threads, 78 KB (2 per CU); 2: 256 threads, 150 KB (1 per CU), which
+// defers nothing (global-memory sweeps for what does not fit, vectors in HBM when even
+// they exceed 48 KB).
+constexpr int kTierNT[3] = {128, 256, 256};
+constexpr size_t kTierLds[3] = {32 * 1024 - 64, 78 * 1024, 150 * 1024};
 constexpr size_t kVecLdsMax = 48 * 1024;
 
 size_t vec_bytes(int mcap, int ncap, int nt) { return VLayout(mcap, ncap, nt).total; }
 bool vec_in_lds(int mcap, int ncap, int nt) { return vec_bytes(mcap, ncap, nt) <= kVecLdsMax; }
 
 // (stage_cap, total dynamic LDS) of a launch
-void lds_plan(bool first, int mcap, int ncap, int64_t& stage_cap, size_t& lds) {
-  const int nt = first ? kFirstNT : 256;
+void lds_plan(int tier, int mcap, int ncap, int64_t& stage_cap, size_t& lds) {
+  const int nt = kTierNT[tier];
   const size_t vb = vec_bytes(mcap, ncap, nt);
   if (!vec_in_lds(mcap, ncap, nt)) { stage_cap = 0; lds = 0; return; }
-  const size_t budget = first ? kFirstLds : kBlockLds;
+  const size_t budget = kTierLds[tier];
   stage_cap = budget > vb ? (int64_t)(budget - vb) : 0;
   // never more than the largest LP of the bin can use (FP64 tableau, k = min(m, n))
   const int64_t most = (int64_t)stage_bytes(mcap, mcap < ncap ? mcap : ncap, 8);
   if (stage_cap > most) stage_cap = most;
   lds = vb + (size_t)stage_cap;
 }
-const void* kernel_of(bool first) {
-  return first ? reinterpret_cast<const void*>(&k_verify<kFirstNT>)
-               : reinterpret_cast<const void*>(&k_verify<256>);
+const void* kernel_of(int tier) {
+  return kTierNT[tier] == 128 ? reinterpret_cast<const void*>(&k_verify<128>)
+                              : reinterpret_cast<const void*>(&k_verify<256>);
 }
 }  // namespace
 
-bool verify_use_wave(int mcap, int ncap, int tsz) {
+bool verify_tier_used(int tier, int mcap, int ncap, int tsz) {
   int64_t sc;
   size_t lds;
-  lds_plan(true, mcap, ncap, sc, lds);
-  // worth a first pass when LPs with a dozen basic structurals fit (LPs that do not are
-  // handed on after their bookkeeping)
-  return vec_in_lds(mcap, ncap, kFirstNT) && sc >= (int64_t)stage_bytes(mcap, 12, tsz);
+  lds_plan(tier, mcap, ncap, sc, lds);
+  const int kmax = mcap < ncap ? mcap : ncap;
+  if (tier == 0)  // worth a first pass when LPs with a dozen basic structurals fit
+    return vec_in_lds(mcap, ncap, kTierNT[0]) && sc >= (int64_t)stage_bytes(mcap, 12, tsz);
+  if (tier == 1)  // worth it when a quarter of the largest blocks fit
+    return vec_in_lds(mcap, ncap, 256) && sc >= (int64_t)stage_bytes(mcap, kmax / 4, tsz);
+  return true;
 }
 
-int verify_max_blocks_per_cu(bool first, int mcap, int ncap) {
+bool verify_tier_covers(int tier, int mcap, int ncap, int tsz) {
+  int64_t sc;
+  size_t lds;
+  lds_plan(tier, mcap, ncap, sc, lds);
+  return vec_in_lds(mcap, ncap, kTierNT[tier]) &&
+         sc >= (int64_t)stage_bytes(mcap, mcap < ncap ? mcap : ncap, tsz);
+}
+
+int verify_max_blocks_per_cu(int tier, int mcap, int ncap) {
   int nb = 0;
   int64_t sc;
   size_t lds;
-  lds_plan(first, mcap, ncap, sc, lds);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_of(first), first ? kFirstNT : 256,
-                                                   lds) != hipSuccess)
+  lds_plan(tier, mcap, ncap, sc, lds);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_of(tier), kTierNT[tier], lds) !=
+      hipSuccess)
     return 0;
   return nb;
 }
 
-// HBM vectors per workgroup of the 256-thread kernel when they exceed the LDS share
-// (0: LDS); the first-pass kernel always keeps them in LDS
+// HBM vectors per workgroup of the last tier when they exceed the LDS share (0: LDS)
 size_t verify_ws_stride(int mcap, int ncap) {
   return vec_in_lds(mcap, ncap, 256) ? 0 : (vec_bytes(mcap, ncap, 256) + 255) & ~(size_t)255;
 }
 
 hipError_t launch_verify(const VerifyLaunch& V, hipStream_t st) {
-  const int nt = V.wave ? kFirstNT : 256;
-  if (V.wave && (!V.defer_list || !V.defer_count || !vec_in_lds(V.mcap, V.ncap, nt)))
+  if (V.tier < 0 || V.tier > 2) return hipErrorInvalidValue;
+  const int nt = kTierNT[V.tier];
+  const bool defers = V.tier < 2 && V.defer_list;
+  if (V.tier < 2 && !vec_in_lds(V.mcap, V.ncap, nt)) return hipErrorInvalidValue;
+  // a tier that may defer must be given a list, unless every LP of the bin fits it
+  if (V.tier < 2 && (!V.defer_list || !V.defer_count) &&
+      !verify_tier_covers(V.tier, V.mcap, V.ncap, V.fp32_source ? 4 : 8))
     return hipErrorInvalidValue;
   int64_t stage_cap;
   size_t lds;
-  lds_plan(V.wave, V.mcap, V.ncap, stage_cap, lds);
+  lds_plan(V.tier, V.mcap, V.ncap, stage_cap, lds);
   const int lds_scratch = vec_in_lds(V.mcap, V.ncap, nt) ? 1 : 0;
   const int64_t vb = (int64_t)vec_bytes(V.mcap, V.ncap, nt);
   char* ws = (char*)V.ws;
-  const int64_t stride = V.wave ? 0 : (int64_t)verify_ws_stride(V.mcap, V.ncap);
-  int32_t* dl = V.wave ? V.defer_list : nullptr;
-  int32_t* dc = V.wave ? V.defer_count : nullptr;
+  const int64_t stride = V.tier < 2 ? 0 : (int64_t)verify_ws_stride(V.mcap, V.ncap);
+  int32_t* dl = defers ? V.defer_list : nullptr;
+  int32_t* dc = defers ? V.defer_count : nullptr;
   void* args[] = {(void*)&V.desc, (void*)&V.pool, (void*)&V.spool, (void*)&V.idx,
                   (void*)&V.count, (void*)&V.count_dev, (void*)&V.out, (void*)&V.chk,
                   (void*)&V.tol, (void*)&V.fp32_source, (void*)&ws, (void*)&stride,
                   (void*)&lds_scratch, (void*)&vb, (void*)&stage_cap, (void*)&V.queue,
                   (void*)&V.fail_list, (void*)&V.fail_count, (void*)&dl, (void*)&dc};
-  return hipLaunchKernel(kernel_of(V.wave), dim3(V.grid), dim3(nt), args, lds, st);
+  return hipLaunchKernel(kernel_of(V.tier), dim3(V.grid), dim3(nt), args, lds, st);
 }
 
 }  // namespace blpg

@@ -62,9 +62,9 @@ TileVariant tile_variant(int precision, int v);
 int tile_max_blocks_per_cu(int precision, int v);
 hipError_t launch_tile(const TileLaunch& L, hipStream_t st);
 
-// FP64 check of final bases (kernel_verify.hip).  `wave`: the one-wavefront kernel, which
-// defers LPs whose staged blocks exceed its LDS budget to defer_list (the four-wavefront
-// kernel then takes that list).
+// FP64 check of final bases (kernel_verify.hip), in up to three tiers of growing LDS per
+// LP; a tier below 2 given a defer_list hands the LPs whose staged blocks it cannot hold
+// to that list, which the next tier takes.
 struct VerifyLaunch {
   const LpDesc* desc;
   const double* pool;
@@ -76,19 +76,20 @@ struct VerifyLaunch {
   Check chk;
   double tol;          // relative KKT tolerance
   int fp32_source;     // 1: the tableaux are FP32 (else FP64)
-  bool wave = false;
-  void* ws;            // grid * verify_ws_stride bytes of scratch when it does not fit LDS
+  int tier = 2;
+  void* ws;            // tier 2: grid * verify_ws_stride bytes of scratch (vectors beyond LDS)
   int mcap, ncap;      // largest m, n
   int grid;
   int32_t* queue;
   int32_t* fail_list;  // LP ids that failed the check (appended)
   int32_t* fail_count;
-  int32_t* defer_list = nullptr;  // wave kernel: LPs handed to the four-wavefront kernel
+  int32_t* defer_list = nullptr;
   int32_t* defer_count = nullptr;
 };
 hipError_t launch_verify(const VerifyLaunch& L, hipStream_t st);
-bool verify_use_wave(int mcap, int ncap, int tsz);  // tsz: tableau element bytes
-int verify_max_blocks_per_cu(bool wave, int mcap, int ncap);
+bool verify_tier_used(int tier, int mcap, int ncap, int tsz);    // tsz: tableau element bytes
+bool verify_tier_covers(int tier, int mcap, int ncap, int tsz);  // every LP of the bin fits
+int verify_max_blocks_per_cu(int tier, int mcap, int ncap);
 size_t verify_ws_stride(int mcap, int ncap);  // 0: vectors in LDS
 
 // small helpers (kernel_util.hip)
