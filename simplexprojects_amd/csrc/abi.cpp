@@ -620,7 +620,9 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
   std::vector<int64_t> first(nb, 0);
   for (int i = 1; i < nb; ++i) first[i] = first[i - 1] + (int64_t)B->bins[i - 1].ids.size();
   bool used[kStreams] = {};
-  for (int i = 0; i < nb; ++i) {
+  // enqueued largest first: persistent grids take the CUs in launch order, so the bin that
+  // sets the critical path starts at once and the small bins fill the CUs it frees
+  for (int i : order) {
     const Bin& b = B->bins[i];
     const int32_t cnt = (int32_t)b.ids.size();
     const int64_t pos = first[i];
