@@ -260,8 +260,9 @@ int blpg_server_poll(blpg_server *server, blpg_server_result *out, int32_t max, 
                      int32_t *n);
 int blpg_server_stats_get(blpg_server *server, blpg_server_stats *stats);
 /* Load generator: submits lps[k] at arrival_us[k] (microseconds after the call, ascending)
- * from a pacing thread, retrying admission, and waits until all completed; latencies count
- * from the scheduled arrival.  Results per LP go to out[k] (optional). */
+ * from up to 8 client threads (arrival k on thread k mod P), retrying admission, and waits
+ * until all completed; latencies count from the scheduled arrival.  Results per LP go to
+ * out[k] (optional). */
 int blpg_server_replay(blpg_server *server, const blpg_lp *lps, const double *arrival_us,
                        int64_t count, blpg_server_result *out);
 

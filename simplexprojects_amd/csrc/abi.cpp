@@ -103,11 +103,9 @@ int blpg_create(blpg_ctx** out, const blpg_options* opt) {
     return BLPG_E_HIP;
   }
   ctx->tuner.init(ctx->props);
-  for (int i = 0; i < blpg_ctx::kAux; ++i)
-    if (hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking) != hipSuccess) {
-      blpg_destroy(ctx);
-      return BLPG_E_HIP;
-    }
+  // the aux streams are created when a batch first needs them (a one-bin batch, such as a
+  // serving arena, never does: its context then holds a single stream, and the process's
+  // few hardware queues are not taken by idle streams)
   if (hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess) {
     blpg_destroy(ctx);
     return BLPG_E_HIP;
@@ -268,6 +266,11 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const int64_t n = count;
   const int nb = (int)B->bins.size();
+  for (int i = 0; i < std::min(nb - 1, blpg_ctx::kAux); ++i)
+    if (!ctx->aux[i] && hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking) != hipSuccess) {
+      delete B;
+      return fail(ctx, BLPG_E_HIP, "stream creation failed");
+    }
   size_t off = 0;
   const size_t o_desc = off; off += al(sizeof(LpDesc) * n);
   const size_t o_pool = off; off += al(sizeof(double) * B->pool_elems);

@@ -4,12 +4,15 @@
 //
 // Per LP shape (m, n, with / without senses) the server keeps `arenas` device batches of
 // `max_batch` slots each (fixed pool offsets per slot, a device-side LP count), each with
-// its own context and stream.  The worker thread
+// its own context and stream, and a pinned mirror of its input pool.  A request is copied
+// once, by the submitting thread, straight into the next free slot of its shape's filling
+// arena (the slot is reserved under the lock, the copy runs outside it).  When no arena
+// can take it (all busy or full) it waits in the shape's overflow queue, which the worker
+// stages into the next free arena first, so requests launch in arrival order.  The worker
 //   - completes finished arenas (hipStreamQuery), copying only the used slots' results;
-//   - launches a free arena when its shape's queue is full, when the oldest request has
-//     waited max_wait_us, or when no arena of that shape is busy: the queued LPs are
-//     staged into a pinned mirror of the arena's pool, copied with one H2D per pool, the
-//     count is written to the device, and the arena's graph is replayed.
+//   - launches the filling arena when it is full, when its oldest request has waited
+//     max_wait_us, or when no arena of that shape is busy, once every reserved slot has
+//     been copied: one H2D per pool, the count to the device, the arena's graph replayed.
 // Admission: arena bytes + bytes of admitted, unpolled requests <= hbm_budget.
 #include <hip/hip_runtime.h>
 
@@ -34,15 +37,21 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-struct Req {
+// what completion needs of a request
+struct Meta {
   int64_t ticket = 0;
   int m = 0, n = 0;
-  std::vector<double> data;  // A (m*n) | b (m) | c (n)
-  std::vector<int8_t> sense;
   double* x = nullptr;
   int32_t* basis = nullptr;
   Clock::time_point t0;
   int64_t bytes = 0;
+};
+
+// a request waiting in the overflow queue (its data copied to the heap)
+struct Req {
+  Meta meta;
+  std::vector<double> data;  // A (m*n) | b (m) | c (n)
+  std::vector<int8_t> sense;
 };
 
 struct Arena {
@@ -53,7 +62,9 @@ struct Arena {
   int32_t* h_count = nullptr;
   int32_t* d_count = nullptr;
   int64_t pool_elems = 0, spool_elems = 0;
-  std::vector<Req> inflight;
+  std::vector<Meta> filled;     // reserved slots 0..filled.size()-1 (not launched yet)
+  std::atomic<int> copied{0};   // reserved slots whose data is in h_pool
+  std::vector<Meta> inflight;
   bool busy = false;
   bool failed = false;  // the launch failed: its requests complete as undecided
   size_t bytes = 0;  // device bytes
@@ -62,7 +73,8 @@ struct Arena {
 struct Shape {
   int m = 0, n = 0;
   bool sense = false;
-  std::deque<Req> queue;
+  int fill = -1;          // arena taking new requests (-1: none chosen)
+  std::deque<Req> queue;  // overflow, in arrival order
   std::vector<std::unique_ptr<Arena>> arenas;
 };
 
@@ -98,7 +110,8 @@ void destroy_arena(Arena& a) {
   if (a.h_sense) (void)hipHostFree(a.h_sense);
   if (a.h_count) (void)hipHostFree(a.h_count);
   if (a.ctx) blpg_destroy(a.ctx);
-  a = Arena{};
+  a.ctx = nullptr; a.B = nullptr; a.h_pool = nullptr; a.h_sense = nullptr;
+  a.h_count = nullptr; a.d_count = nullptr;
 }
 
 // An arena: a device batch of `cap` distinct slots of shape (m, n), uploaded once with
@@ -146,24 +159,23 @@ int make_arena(blpg_server* S, Shape& sh, Arena& a) {
   return BLPG_OK;
 }
 
-// Stage `reqs` into slots 0..k-1 of the arena and replay its graph (asynchronous).
-int launch(Arena& a, const std::vector<Req>& reqs) {
+// Copies one LP into slot `slot` of the arena's pinned pool mirror.
+void copy_slot(Arena& a, int slot, const double* A, const double* b, const double* c,
+               const int8_t* sense) {
+  const LpDesc& d = a.B->desc[slot];
+  const size_t mn = (size_t)d.m * d.n;
+  std::memcpy(a.h_pool + d.a, A, sizeof(double) * mn);
+  std::memcpy(a.h_pool + d.b, b, sizeof(double) * d.m);
+  std::memcpy(a.h_pool + d.c, c, sizeof(double) * d.n);
+  if (a.h_sense && d.s >= 0 && sense) std::memcpy(a.h_sense + d.s, sense, d.m);
+}
+
+// Uploads slots 0..k-1 of the arena and replays its graph (asynchronous).
+int fire(Arena& a, int k) {
   blpg_batch* B = a.B;
-  const int k = (int)reqs.size();
-  int64_t end = 0, send = 0;
-  for (int s = 0; s < k; ++s) {
-    const LpDesc& d = B->desc[s];
-    const Req& r = reqs[s];
-    const size_t mn = (size_t)d.m * d.n;
-    std::memcpy(a.h_pool + d.a, r.data.data(), sizeof(double) * mn);
-    std::memcpy(a.h_pool + d.b, r.data.data() + mn, sizeof(double) * d.m);
-    std::memcpy(a.h_pool + d.c, r.data.data() + mn + d.m, sizeof(double) * d.n);
-    end = std::max<int64_t>(end, d.c + d.n);
-    if (a.h_sense && d.s >= 0) {
-      std::memcpy(a.h_sense + d.s, r.sense.data(), d.m);
-      send = std::max<int64_t>(send, d.s + d.m);
-    }
-  }
+  const LpDesc& last = B->desc[k - 1];
+  const int64_t end = last.c + last.n;
+  const int64_t send = (a.h_sense && last.s >= 0) ? last.s + last.m : 0;
   *a.h_count = k;
   hipStream_t st = a.ctx->stream;
   if (hipMemcpyAsync(B->d_pool, a.h_pool, sizeof(double) * end, hipMemcpyHostToDevice, st) != hipSuccess ||
@@ -182,7 +194,7 @@ int finish(blpg_server* S, Arena& a) {
   const auto now = Clock::now();
   std::lock_guard<std::mutex> lk(S->mu);
   for (int64_t s = 0; s < k; ++s) {
-    Req& r = a.inflight[s];
+    Meta& r = a.inflight[s];
     blpg_server_result o{};
     o.ticket = r.ticket;
     o.status = rc ? BLPG_UNDECIDED : R.status[s];
@@ -218,49 +230,87 @@ void worker_loop(blpg_server* S) {
           if (a->busy && hipStreamQuery(a->ctx->stream) == hipSuccess) ready.push_back(a.get());
     }
     for (Arena* a : ready) { finish(S, *a); progress = true; }
-    // launches
-    std::vector<std::pair<Arena*, std::vector<Req>>> todo;
-    std::vector<std::pair<Shape*, Arena*>> fresh;  // shapes that need a new arena
+    // launches: the filling arena of each shape, and overflow requests (staged here)
+    std::vector<std::pair<Arena*, std::vector<Req>>> staged;
+    std::vector<Arena*> ready_fill;
     {
       std::lock_guard<std::mutex> lk(S->mu);
       const auto now = Clock::now();
       for (auto& kv : S->shapes) {
         Shape& sh = *kv.second;
-        if (sh.queue.empty()) continue;
-        Arena* freea = nullptr;
         bool any_busy = false;
-        for (auto& a : sh.arenas) {
-          if (a->busy) any_busy = true;
-          else if (!freea) freea = a.get();
+        for (auto& a : sh.arenas) any_busy = any_busy || a->busy;
+        // the filling arena (its requests are older than any in the overflow queue)
+        if (sh.fill >= 0) {
+          Arena* fa = sh.arenas[sh.fill].get();
+          const bool full = (int)fa->filled.size() >= S->cfg.max_batch;
+          if (!fa->busy && !fa->filled.empty() &&
+              (full || !sh.queue.empty() || !any_busy ||
+               now - fa->filled.front().t0 >= max_wait) &&
+              fa->copied.load(std::memory_order_acquire) == (int)fa->filled.size()) {
+            fa->busy = true;
+            any_busy = true;
+            fa->inflight = std::move(fa->filled);
+            fa->filled.clear();
+            fa->copied.store(0, std::memory_order_relaxed);
+            sh.fill = -1;
+            ready_fill.push_back(fa);
+          }
         }
-        const bool full = (int)sh.queue.size() >= S->cfg.max_batch;
-        const bool aged = now - sh.queue.front().t0 >= max_wait;
-        if (!(full || aged || !any_busy)) continue;
-        if (!freea) continue;
-        const int k = std::min<int>((int)sh.queue.size(), S->cfg.max_batch);
-        std::vector<Req> reqs;
-        reqs.reserve(k);
-        for (int i = 0; i < k; ++i) {
-          reqs.push_back(std::move(sh.queue.front()));
-          sh.queue.pop_front();
+        // overflow requests into a free, empty arena
+        if (!sh.queue.empty()) {
+          Arena* freea = nullptr;
+          for (auto& a : sh.arenas)
+            if (!a->busy && a->filled.empty() && !freea) freea = a.get();
+          const bool full = (int)sh.queue.size() >= S->cfg.max_batch;
+          const bool aged = now - sh.queue.front().meta.t0 >= max_wait;
+          if (freea && (full || aged || !any_busy)) {
+            const int k = std::min<int>((int)sh.queue.size(), S->cfg.max_batch);
+            std::vector<Req> reqs;
+            reqs.reserve(k);
+            for (int i = 0; i < k; ++i) {
+              reqs.push_back(std::move(sh.queue.front()));
+              sh.queue.pop_front();
+            }
+            staged.push_back({freea, std::move(reqs)});
+            freea->busy = true;  // reserved
+          }
         }
-        todo.push_back({freea, std::move(reqs)});
-        freea->busy = true;  // reserved
       }
     }
-    for (auto& t : todo) {
-      const int k = (int)t.second.size();
-      const int rc = launch(*t.first, t.second);
+    auto record = [&](Arena* a, int k, int rc) {
       std::lock_guard<std::mutex> lk(S->mu);
-      t.first->inflight = std::move(t.second);  // busy stays set: finished by the poll above
       if (rc) {
         S->err = "launch failed";
-        t.first->failed = true;
+        a->failed = true;
       } else {
         ++S->batches;
         S->batched_lps += k;
       }
       progress = true;
+    };
+    for (auto& t : staged) {
+      Arena* a = t.first;
+      const int k = (int)t.second.size();
+      std::vector<Meta> metas;
+      metas.reserve(k);
+      for (int s = 0; s < k; ++s) {
+        const Req& r = t.second[s];
+        const size_t mn = (size_t)r.meta.m * r.meta.n;
+        copy_slot(*a, s, r.data.data(), r.data.data() + mn, r.data.data() + mn + r.meta.m,
+                  r.sense.empty() ? nullptr : r.sense.data());
+        metas.push_back(r.meta);
+      }
+      const int rc = fire(*a, k);
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        a->inflight = std::move(metas);  // busy stays set: finished by the poll above
+      }
+      record(a, k, rc);
+    }
+    for (Arena* a : ready_fill) {
+      const int k = (int)a->inflight.size();
+      record(a, k, fire(*a, k));
     }
     if (!progress) {
       std::unique_lock<std::mutex> lk(S->mu);
@@ -322,14 +372,9 @@ int submit_at(blpg_server* S, const blpg_lp* lp, double* x, int32_t* basis, Cloc
     if (!std::isfinite(lp->c[j])) return BLPG_E_INVALID;
   const bool has_sense = lp->sense != nullptr;
   const auto key = std::make_tuple(m, n, (int)has_sense);
-  Req r;
+  Meta r;
   r.m = m;
   r.n = n;
-  r.data.resize(mn + m + n);
-  std::memcpy(r.data.data(), lp->A, sizeof(double) * mn);
-  std::memcpy(r.data.data() + mn, lp->b, sizeof(double) * m);
-  std::memcpy(r.data.data() + mn + m, lp->c, sizeof(double) * n);
-  if (has_sense) r.sense.assign(lp->sense, lp->sense + m);
   r.x = x;
   r.basis = basis;
   r.t0 = t0;
@@ -377,8 +422,40 @@ int submit_at(blpg_server* S, const blpg_lp* lp, double* x, int32_t* basis, Cloc
   if (ticket) *ticket = r.ticket;
   S->pending_bytes += r.bytes;
   ++S->submitted;
-  it->second->queue.push_back(std::move(r));
-  lk.unlock();
+  Shape& sh = *it->second;
+  // the filling arena, or a free empty one (only while nothing waits in the overflow)
+  Arena* a = nullptr;
+  int slot = -1;
+  if (sh.queue.empty()) {
+    if (sh.fill >= 0) {
+      Arena* fa = sh.arenas[sh.fill].get();
+      if (!fa->busy && (int)fa->filled.size() < S->cfg.max_batch) a = fa;
+    }
+    if (!a)
+      for (size_t i = 0; i < sh.arenas.size(); ++i)
+        if (!sh.arenas[i]->busy && sh.arenas[i]->filled.empty()) {
+          sh.fill = (int)i;
+          a = sh.arenas[i].get();
+          break;
+        }
+  }
+  if (a) {
+    slot = (int)a->filled.size();
+    a->filled.push_back(r);
+    lk.unlock();
+    copy_slot(*a, slot, lp->A, lp->b, lp->c, lp->sense);  // outside the lock
+    a->copied.fetch_add(1, std::memory_order_release);
+  } else {
+    Req q;
+    q.meta = r;
+    q.data.resize(mn + m + n);
+    std::memcpy(q.data.data(), lp->A, sizeof(double) * mn);
+    std::memcpy(q.data.data() + mn, lp->b, sizeof(double) * m);
+    std::memcpy(q.data.data() + mn + m, lp->c, sizeof(double) * n);
+    if (has_sense) q.sense.assign(lp->sense, lp->sense + m);
+    sh.queue.push_back(std::move(q));
+    lk.unlock();
+  }
   S->cv_work.notify_one();
   return BLPG_OK;
 }
@@ -434,23 +511,29 @@ int blpg_server_replay(blpg_server* S, const blpg_lp* lps, const double* arrival
   std::vector<int64_t> tickets(count, -1);
   std::atomic<int> err{0};
   const auto start = Clock::now();
-  std::thread pacer([&] {
-    for (int64_t k = 0; k < count && !err.load(); ++k) {
+  // several client threads (arrival k goes to thread k mod P), each submitting at the
+  // scheduled times: the copy of a request into its arena happens on the client's thread
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int P = (int)std::max(1u, std::min(8u, hw > 2 ? hw / 2 : 1u));
+  auto pace = [&](int t) {
+    for (int64_t k = t; k < count && !err.load(); k += P) {
       const auto due = start + std::chrono::nanoseconds((int64_t)(arrival_us[k] * 1000.0));
       while (Clock::now() < due) std::this_thread::yield();
       for (;;) {
-        int64_t t = -1;
+        int64_t tk = -1;
         // latency counts from the scheduled arrival
-        const int rc = submit_at(S, &lps[k], nullptr, nullptr, due, &t);
+        const int rc = submit_at(S, &lps[k], nullptr, nullptr, due, &tk);
         if (rc == BLPG_OK) {
-          tickets[k] = t;
+          tickets[k] = tk;
           break;
         }
         if (rc != BLPG_E_BUDGET) { err.store(rc); break; }
         std::this_thread::yield();
       }
     }
-  });
+  };
+  std::vector<std::thread> pacers;
+  for (int t = 0; t < P; ++t) pacers.emplace_back(pace, t);
   std::map<int64_t, blpg_server_result> got;
   std::vector<blpg_server_result> buf(1024);
   while ((int64_t)got.size() < count && !err.load()) {
@@ -458,7 +541,7 @@ int blpg_server_replay(blpg_server* S, const blpg_lp* lps, const double* arrival
     blpg_server_poll(S, buf.data(), (int32_t)buf.size(), 1000, &n);
     for (int i = 0; i < n; ++i) got[buf[i].ticket] = buf[i];
   }
-  pacer.join();
+  for (auto& th : pacers) th.join();
   if (out)
     for (int64_t k = 0; k < count; ++k) {
       auto it = got.find(tickets[k]);
