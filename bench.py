@@ -40,7 +40,7 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
 # same counters on a plain 256 MiB device copy (FETCH x2.000, WRITE x1.000); the commands and
 # raw counts are in profiles/tile_fp32_105x103_mixed_batch.md.
-PROFILED_TRAFFIC = {("tile", 4, "fp32", 105, 103, 1024): 92.65e6}
+PROFILED_TRAFFIC = {("tile", 4, "fp32", 105, 103, 1024): 152.3e6}
 
 
 def build_batch(sa):
@@ -120,7 +120,7 @@ def warmstart_leg(sa, steps, warmup, do_tune=True):
     t0 = time.perf_counter()
     rt = S.solve(train)
     t_label = time.perf_counter() - t0
-    ws = sa.WarmStart(S, nf, m, n, seed=4, steps=300)
+    ws = sa.WarmStart(S, nf, m, n, seed=4, steps=1000, lr=3e-3, weight_decay=1e-2)
     t0 = time.perf_counter()
     loss = ws.train(train, rt.basis)
     t_train = time.perf_counter() - t0
