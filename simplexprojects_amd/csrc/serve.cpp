@@ -99,6 +99,8 @@ struct blpg_server {
 
 namespace {
 
+constexpr int64_t kLargeTableau = 150 * 150;  // (m+2)(n+2) from which an arena is masked
+
 int64_t req_bytes(int m, int n, bool sense) {
   return 8 * ((int64_t)m * n + m + n) + (sense ? m : 0) + 12 * (int64_t)(m + n) + 64;
 }
@@ -125,6 +127,21 @@ int make_arena(blpg_server* S, Shape& sh, Arena& a) {
   o.use_graph = 1;
   int rc = blpg_create(&a.ctx, &o);
   if (rc) return rc;
+  // Arenas of large shapes (LPs that hold a CU for milliseconds) run on three quarters of
+  // the CUs, so batches of small shapes always find free CUs instead of queueing behind
+  // them: a stream with a CU mask that leaves out every fourth group of 8 CUs (spread over
+  // the XCDs however CU indices map to them).
+  if ((int64_t)(sh.m + 2) * (sh.n + 2) >= kLargeTableau) {
+    const int ncu = a.ctx->props.multiProcessorCount;
+    std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+    for (int cu = 0; cu < ncu; ++cu)
+      if ((cu / 8) % 4 != 3) mask[cu / 32] |= 1u << (cu % 32);
+    hipStream_t masked = nullptr;
+    if (hipExtStreamCreateWithCUMask(&masked, (uint32_t)ncu, mask.data()) == hipSuccess) {
+      (void)hipStreamDestroy(a.ctx->stream);
+      a.ctx->stream = masked;
+    }
+  }
   const std::string tune = S->parent->tuner.export_text();
   if (!tune.empty()) (void)blpg_tune_import(a.ctx, tune.c_str());
   const int m = sh.m, n = sh.n;
