@@ -236,6 +236,12 @@ def gen_config(config: int, seed: int, m: int, n: int, count: int, shape_idx: in
     if rc != 0:
         raise BlpgError(rc, "blpg_gen_config failed")
     use_sense = config == 3
+    if config == 4 and count > 0 and np.array_equal(A, np.broadcast_to(A[0], A.shape)):
+        # a parametric family shares one A: every LP refers to the same array, so a batch
+        # upload stores it once (the pool deduplicates by pointer, as for shared-A batches)
+        A0 = A[0].copy()
+        return [LP(A0, b[k], c[k], None, family=shape_idx, flags=int(f[k]))
+                for k in range(count)]
     return [LP(A[k], b[k], c[k], s[k] if use_sense else None,
                family=shape_idx if config == 4 else -1, flags=int(f[k])) for k in range(count)]
 
