@@ -126,6 +126,8 @@ struct StreamTab {
     __syncthreads();
     return uniform(bs[i]);
   }
+  __device__ __forceinline__ int bs_now(int i) { return bs_of(i); }
+  __device__ __forceinline__ int warm_at(const int32_t* warm, int ci) const { return warm[ci]; }
 
   __device__ __forceinline__ int pricing(int orow, bool bland, T eps_d) {
     T bv = inf_of<T>();

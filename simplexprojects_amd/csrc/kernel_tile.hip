@@ -58,6 +58,8 @@ struct TileTab {
   int* nb;
   uint8_t* redund;
   uint8_t* target;  // target[var] = 1: var belongs to the warm-start basis
+  int* wl;          // the warm-start list (m entries), read by the crash in order
+  int* colof;       // column of each nonbasic variable, -1 while it is basic
   double* s_am;     // per-wave max |A_ij| (NW entries)
   Decision<T>* slots;  // 2 broadcast slots
 
@@ -125,12 +127,15 @@ struct TileTab {
     if (lane == 0) s_am[tid >> 6] = am;
     for (int i = tid; i < RCAP; i += NT) { bs[i] = (i >= 2 && i < R) ? n + i - 2 : -1; redund[i] = 0; }
     for (int j = tid; j < CCAP; j += NT) nb[j] = (j == 0 || j >= C) ? -1 : (j <= n ? j - 1 : n + m);
+    for (int x = tid; x < RCAP + CCAP; x += NT)
+      colof[x] = x < n ? x + 1 : (x == n + m ? C - 1 : -1);
     publish_rhs();
     col_ready = -1;
     __syncthreads();
   }
 
-  // marks the warm-start targets; false when the list holds no valid variable id
+  // marks the warm-start targets and keeps the list in LDS; false when the list holds no
+  // valid variable id
   __device__ __forceinline__ bool set_target(const int32_t* warm) {
     for (int x = tid; x < RCAP + CCAP; x += NT) target[x] = 0;
     __syncthreads();
@@ -138,10 +143,17 @@ struct TileTab {
     if (warm)
       for (int i = tid; i < m; i += NT) {
         const int x = warm[i];
+        wl[i] = x;
         if (x >= 0 && x < n + m) { target[x] = 1; any = 1; }
       }
     return __syncthreads_or(any) != 0;
   }
+
+  // entry ci of the warm-start list (from LDS; uniform)
+  __device__ __forceinline__ int warm_at(const int32_t*, int ci) const { return uniform(wl[ci]); }
+
+  // basic variable of row i when no pivot has happened since the last barrier
+  __device__ __forceinline__ int bs_now(int i) const { return uniform(bs[i]); }
 
   // the TR owners of column s store it in colbuf
   __device__ __forceinline__ void publish_col(int s) {
@@ -266,7 +278,13 @@ struct TileTab {
         }
       }
     }
-    if (tid == 0) { const int t = bs[r]; bs[r] = nb[s]; nb[s] = t; }
+    if (tid == 0) {
+      const int t = bs[r], e = nb[s];
+      bs[r] = e;
+      nb[s] = t;
+      colof[e] = -1;
+      colof[t] = s;
+    }
     __syncthreads();
     T col[RPT], prow[CPT];
 #pragma unroll
@@ -452,18 +470,9 @@ struct TileTab {
     return bp;
   }
 
-  __device__ __forceinline__ int find_nonbasic(int var) {
-    __syncthreads();
-    T bv = inf_of<T>();
-    int bk = INT_MAX, bp = -1;
-    if (wv == 0) {
-      ArgMin<T> am;
-      for (int j = 1 + lane; j < C; j += 64) am.add(nb[j] == var, T(0), j, j);
-      wave_finish(am, bv, bk, bp);
-    }
-    bcast(bv, bk, bp);
-    return bp;
-  }
+  // column of nonbasic variable var (-1 if basic), from the map the pivots keep.  No barrier:
+  // a pivot writes the map before its own barrier, and only pivots write it.
+  __device__ __forceinline__ int find_nonbasic(int var) { return uniform(colof[var]); }
 
   __device__ __forceinline__ int nb_of(int j) {
     __syncthreads();
@@ -525,6 +534,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) 
   __shared__ T colbuf[Tab::RCAP], rhsbuf[Tab::RCAP], rowbuf[Tab::CCAP], scal[2];
   __shared__ int bs[Tab::RCAP], nb[Tab::CCAP];
   __shared__ uint8_t redund[Tab::RCAP], target[Tab::RCAP + Tab::CCAP];
+  __shared__ int wl[Tab::RCAP], colof[Tab::RCAP + Tab::CCAP];
   __shared__ Decision<T> slots[2];
   __shared__ double s_am[NW];
   __shared__ int s_q;
@@ -538,6 +548,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) 
   t.phase = 0;
   t.colbuf = colbuf; t.rhsbuf = rhsbuf; t.rowbuf = rowbuf; t.scal = scal;
   t.bs = bs; t.nb = nb; t.redund = redund; t.target = target;
+  t.wl = wl; t.colof = colof;
   t.slots = slots;
   if (count_dev) count = *count_dev;
   for (;;) {

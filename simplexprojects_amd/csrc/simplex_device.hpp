@@ -295,7 +295,7 @@ __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* 
     if (stage == kStCrash) {
       int v = -1;
       while (ci < m && v < 0) {
-        const int w = warm[ci];
+        const int w = t.warm_at(warm, ci);
         ++ci;
         if (w >= 0 && w < n) v = w;
       }
@@ -305,7 +305,7 @@ __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* 
       if (it >= maxit) { status = kStatusIterLimit; break; }
       const int r = t.absmax_row(s, o.eps_p);
       if (r < 0) continue;
-      const int lv = t.bs_of(r);
+      const int lv = t.bs_now(r);  // absmax_row ended at a barrier, no pivot since
       pr = r; ps = s; count_p1 = true;
       kill_after = lv >= n && lv < n + m && sense && sense[lv - n] == 1;
     } else if (stage == kStEq) {
