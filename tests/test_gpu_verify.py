@@ -69,3 +69,13 @@ def test_generated_fp32_check_matches_restatement():
     r = S.solve(lps)
     assert compare(lps, r) >= 80
     S.close()
+
+
+def test_large_lps_global_sweeps_match_restatement():
+    # 205x203 LPs: their staged blocks (k ~ 110) exceed every tier's LDS, so the last tier
+    # runs the sweeps from global memory (vectors in LDS, blocks read in place)
+    lps = sa.gen_config(5, 5, 205, 203, 12, shape_idx=4, batch_count=1024)
+    S = sa.Solver(precision="fp32")
+    r = S.solve(lps)
+    assert compare(lps, r) >= 8
+    S.close()
