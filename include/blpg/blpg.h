@@ -108,7 +108,10 @@ int blpg_set_options(blpg_ctx *ctx, const blpg_options *opt);
 
 /* Blocking batch solve of `count` independent LPs of any sizes.  Replaces
  * Simplex(N_S,..) + setConstratint(A,b) + ComputeLP(C, streams) + getResultAll()
- * (simplex.cu:387-978, called at GPU_Tester.cu:244-259). */
+ * (simplex.cu:387-978, called at GPU_Tester.cu:244-259).  A batch larger than the HBM
+ * budget (or the device's free memory) is solved in consecutive parts that fit, with no
+ * device reset (the reference's multi-batch mode, GPU_Tester.cu:194-239, resets the
+ * device between parts). */
 int blpg_solve_batch(blpg_ctx *ctx, const blpg_lp *lps, int64_t count, blpg_result *out);
 
 /* The reference's batch shape: one A, b (and senses) shared by `count` objectives C

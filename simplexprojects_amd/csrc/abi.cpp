@@ -908,17 +908,35 @@ int blpg_batch_download(blpg_ctx* ctx, blpg_batch* B, blpg_result* out) {
   return BLPG_OK;
 }
 
-int blpg_solve_batch(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_result* out) {
-  if (!ctx) return BLPG_E_INVALID;
-  if (count == 0) return BLPG_OK;
+}  // extern "C"
+
+// Solves lps[0..count) as one device batch; a batch that exceeds the HBM budget or the
+// device's free memory is split in halves, recursively, and the parts solved one after
+// the other (out-of-core batching: no device reset, results in place).  A single LP that
+// does not fit still fails with BLPG_E_BUDGET / BLPG_E_NOMEM.
+static int solve_range(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_result* out) {
   blpg_batch* B = nullptr;
   int rc = blpg_batch_upload(ctx, lps, count, &B);
+  if ((rc == BLPG_E_BUDGET || rc == BLPG_E_NOMEM) && count > 1) {
+    const int64_t h = count / 2;
+    rc = solve_range(ctx, lps, h, out);
+    if (!rc) rc = solve_range(ctx, lps + h, count - h, out + h);
+    return rc;
+  }
   if (rc) return rc;
   rc = blpg_batch_solve(ctx, B);
   if (!rc) rc = blpg_batch_sync(ctx);
   if (!rc) rc = blpg_batch_download(ctx, B, out);
   blpg_batch_free(ctx, B);
   return rc;
+}
+
+extern "C" {
+
+int blpg_solve_batch(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_result* out) {
+  if (!ctx) return BLPG_E_INVALID;
+  if (count == 0) return BLPG_OK;
+  return solve_range(ctx, lps, count, out);
 }
 
 int blpg_tune_set(blpg_ctx* ctx, int32_t precision, int32_t m, int32_t n, int32_t kernel,

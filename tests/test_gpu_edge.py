@@ -143,3 +143,20 @@ def test_fetch_matches_download(precision):
             assert (lps[k].A @ f.x[k] <= lps[k].b + 1e-9).all() and (f.x[k] >= 0).all()
     b.free()
     S.close()
+
+
+def test_batch_beyond_budget_is_solved_in_parts():
+    # out-of-core batching: a batch over the HBM budget is split and solved part by part
+    lps = sa.gen_config(2, 2, 50, 48, 600, shape_idx=0, batch_count=1024)
+    S = sa.Solver(precision="fp64")
+    ref = S.solve(lps)
+    S.close()
+    # ~42 KB of device memory per LP of this shape: 6 MB holds about a quarter of them
+    S2 = sa.Solver(precision="fp64", hbm_budget=6 << 20)
+    r = S2.solve(lps)
+    assert np.array_equal(r.status, ref.status)
+    assert np.array_equal(r.pivots, ref.pivots)
+    assert np.array_equal(r.objective, ref.objective)
+    with pytest.raises(sa.BlpgError):
+        S2.upload(lps)  # one device batch of all of them still exceeds the budget
+    S2.close()
