@@ -22,6 +22,7 @@ namespace blpg {
 template <typename T, int NT>
 struct StreamTab {
   static constexpr int NW = NT / 64;
+  static constexpr bool kBlockCrash = false;  // crash one target per pivot
   T* tab;
   int ld, R, C, m, n;
   T* colbuf;

@@ -28,6 +28,17 @@
 
 namespace blpg {
 
+// lane l's value (l wave-uniform)
+__device__ __forceinline__ float lane_read(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double lane_read(double x, int l) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // wavefront 0's decision, broadcast to the workgroup
 template <typename T>
 struct Decision {
@@ -35,7 +46,9 @@ struct Decision {
   int k, p;
 };
 
-template <typename T, int NW, int TC, int RPT, int CPT>
+// BC: crash in blocks (crash_prepare).  Instantiated only for batches with warm lists, and
+// only where its registers do not spill the tile; cold solves run the BC = false kernel.
+template <typename T, int NW, int TC, int RPT, int CPT, bool BC>
 struct TileTab {
   static constexpr int NT = NW * 64;
   static constexpr int TR = NT / TC;
@@ -62,6 +75,23 @@ struct TileTab {
   int* colof;       // column of each nonbasic variable, -1 while it is basic
   double* s_am;     // per-wave max |A_ij| (NW entries)
   Decision<T>* slots;  // 2 broadcast slots
+
+  // Block crash: KB targets per block.  Their columns and pivot rows take at most 16 KB
+  // of LDS, and wavefront 0 / each column thread holds KB values per row / column in
+  // registers during the block's preparation (more than 8 spills the tile).
+  static constexpr int KB0 = 16384 / ((RCAP + CCAP) * (int)sizeof(T));
+  static constexpr int KB = !BC ? 1 : KB0 < 4 ? 4 : (KB0 > 8 ? 8 : KB0);
+  static constexpr bool kBlockCrash = BC;
+  static constexpr int QR = (RCAP + 63) / 64;  // rows per lane of wavefront 0
+  T* cbM;    // KB x RCAP: column of target u, updated until its own step, then frozen
+  T* cbP;    // KB x CCAP: row of pivot u, updated until its own step, then scaled
+  T* cbRp;   // KB: 1 / pivot element of step u
+  int* cbR;  // KB + 3: pivot row of step u (-1: none); [KB] steps, [KB + 1] next ci,
+             // [KB + 2] pivots made
+  int* cbS;  // KB: column of target u
+  int* cbW;  // KB: the block's pivots in order, packed: row | column << 12 | slot << 24 |
+             // (leaving variable is an equality slack) << 30
+  int blk;   // >= 0: the next pivot() is the crash block's step in this slot
 
   // Called once per state-machine step: the per-thread indices become opaque values, so
   // the compiler recomputes index arithmetic where it is used instead of hoisting every
@@ -155,28 +185,29 @@ struct TileTab {
   // basic variable of row i when no pivot has happened since the last barrier
   __device__ __forceinline__ int bs_now(int i) const { return uniform(bs[i]); }
 
-  // the TR owners of column s store it in colbuf
-  __device__ __forceinline__ void publish_col(int s) {
+  // the TR owners of column s store it in dst (RCAP entries)
+  __device__ __forceinline__ void publish_col(int s, T* dst) {
     const int ls = s / TC, gcs = s - ls * TC;
 #pragma unroll
     for (int l = 0; l < CPT; ++l) {
       if (l == ls) {
         if (gc == gcs) {
 #pragma unroll
-          for (int k = 0; k < RPT; ++k) colbuf[row_of(k)] = v[k][l];
+          for (int k = 0; k < RPT; ++k) dst[row_of(k)] = v[k][l];
         }
       }
     }
   }
+  __device__ __forceinline__ void publish_col(int s) { publish_col(s, colbuf); }
 
-  // the TC owners of row r store it (unscaled) in rowbuf
-  __device__ __forceinline__ void publish_row(int r) {
+  // the TC owners of row r store it (unscaled) in dst (CCAP entries)
+  __device__ __forceinline__ void publish_row(int r, T* dst) {
     const int kr = r / TR, grr = r - kr * TR;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
       if (k == kr && gr == grr) {
 #pragma unroll
-        for (int l = 0; l < CPT; ++l) rowbuf[col_of(l)] = v[k][l];
+        for (int l = 0; l < CPT; ++l) dst[col_of(l)] = v[k][l];
       }
     }
   }
@@ -257,40 +288,52 @@ struct TileTab {
     return bp;
   }
 
+  // One pivot.  With blk >= 0 it is the crash block's step in slot blk, whose column,
+  // scaled row and basis change crash_prepare made: no barrier, every thread reads only
+  // LDS that no one writes until the next barrier.  colbuf / rowbuf are slot 0 of the
+  // block arrays, so both cases read the column and row at slot * RCAP / CCAP.
   __device__ __forceinline__ void pivot(int r, int s) {
-    if (col_ready != s) {
-      __syncthreads();
-      publish_col(s);
-      __syncthreads();
-    }
     const int kr = r / TR, grr = r - kr * TR;
     const int ls = s / TC, gcs = s - ls * TC;
+    int slot = 0;
+    bool prepared = false;
+    if constexpr (BC) {
+      prepared = blk >= 0;
+      if (prepared) slot = blk;
+    }
+    if (!prepared) {
+      if (col_ready != s) {
+        __syncthreads();
+        publish_col(s);
+        __syncthreads();
+      }
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      if (k == kr) {
-        if (gr == grr) {
-          const T rp = T(1) / colbuf[r];
+      for (int k = 0; k < RPT; ++k) {
+        if (k == kr) {
+          if (gr == grr) {
+            const T rp = T(1) / colbuf[r];
 #pragma unroll
-          for (int l = 0; l < CPT; ++l) {
-            const int j = col_of(l);
-            rowbuf[j] = (j == s) ? rp : v[k][l] * rp;
+            for (int l = 0; l < CPT; ++l) {
+              const int j = col_of(l);
+              rowbuf[j] = (j == s) ? rp : v[k][l] * rp;
+            }
           }
         }
       }
+      if (tid == 0) {
+        const int t = bs[r], e = nb[s];
+        bs[r] = e;
+        nb[s] = t;
+        colof[e] = -1;
+        colof[t] = s;
+      }
+      __syncthreads();
     }
-    if (tid == 0) {
-      const int t = bs[r], e = nb[s];
-      bs[r] = e;
-      nb[s] = t;
-      colof[e] = -1;
-      colof[t] = s;
-    }
-    __syncthreads();
     T col[RPT], prow[CPT];
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) col[k] = -colbuf[row_of(k)];
+    for (int k = 0; k < RPT; ++k) col[k] = -colbuf[slot * RCAP + row_of(k)];
 #pragma unroll
-    for (int l = 0; l < CPT; ++l) prow[l] = rowbuf[col_of(l)];
+    for (int l = 0; l < CPT; ++l) prow[l] = rowbuf[slot * CCAP + col_of(l)];
 #pragma unroll
     for (int l = 0; l < CPT; ++l) {
       if (l == ls) {
@@ -313,12 +356,13 @@ struct TileTab {
     }
     publish_rhs();
     col_ready = -1;
+    blk = -1;
   }
 
   // column of largest |T[r][j]| among live columns; -1 if all <= eps_p
   __device__ __forceinline__ int absmax_col(int r, T eps_p) {
     __syncthreads();
-    publish_row(r);
+    publish_row(r, rowbuf);
     __syncthreads();
     T bv = inf_of<T>();
     int bk = INT_MAX, bp = -1;
@@ -358,7 +402,7 @@ struct TileTab {
   // dual simplex entering column for leaving row r (row r published through rowbuf; the
   // objective row is read from the registers of its owners)
   __device__ __forceinline__ int dual_ratio(int r, bool bland, T eps_p, T& theta) {
-    publish_row(r);
+    publish_row(r, rowbuf);
     __syncthreads();
     col_ready = -1;
     T bv = inf_of<T>();
@@ -474,6 +518,185 @@ struct TileTab {
   // a pivot writes the map before its own barrier, and only pivots write it.
   __device__ __forceinline__ int find_nonbasic(int var) { return uniform(colof[var]); }
 
+  // Prepares the next up-to-KB crash targets as one block.  Same targets, rows and
+  // arithmetic as one absmax_row + pivot per target, so the tableau ends bit-identical:
+  //   targets  the warm-list structurals from ci on that are nonbasic, stopping before a
+  //            repeated column (its second turn must see the first one's pivot);
+  //   A        wavefront 0 runs the pivots on the targets' columns alone (cbM): step t
+  //            picks its row from column t as absmax_row does, makes the step's basis
+  //            change, and updates the later columns; column t keeps the values step t
+  //            multiplies by;
+  //   B        one thread per column j runs the pivot rows alone (cbP): row r_t, updated
+  //            by steps < t, becomes step t's scaled pivot row.
+  // The caller then makes the block's pivots (cbW) through pivot() with `blk` set to the
+  // step's slot, which reads the column and row from cbM / cbP and needs no barrier.  Five
+  // barriers per block instead of about five per target.  Returns the number of pivots
+  // (steps without an eligible row make none), or -1 when no target is left (ci = m).
+  __device__ __forceinline__ int blk_word(int j) const { return uniform(cbW[j]); }
+
+  __device__ int crash_prepare(int& ci, const int8_t* sense, T eps_p) {
+    __syncthreads();
+    // targets: wavefront 0 reads the list 64 entries at a time and takes the nonbasic
+    // structurals in order; `sel` on lane u holds target u's column
+    if (wv == 0) {
+      int kc = 0, c = ci, sel = -1;
+      bool stop = false;
+      while (!stop && c < m && kc < KB) {
+        const int idx = c + lane;
+        const int w = idx < m ? wl[idx] : -1;
+        const int s = (w >= 0 && w < n) ? colof[w] : -1;
+        unsigned long long bal = __ballot(s >= 0);
+        int next = c + 64 < m ? c + 64 : m;
+        while (bal != 0ull) {
+          const int l = __ffsll((long long)bal) - 1;
+          const int sl = __builtin_amdgcn_readlane(s, l);
+          if (__ballot(lane < kc && sel == sl) != 0ull) {  // repeated: stop before it
+            stop = true;
+            next = c + l;
+            break;
+          }
+          if (lane == kc) sel = sl;
+          ++kc;
+          bal &= bal - 1ull;
+          if (kc == KB) {
+            next = c + l + 1;
+            break;
+          }
+        }
+        c = next;
+      }
+      if (lane < kc) cbS[lane] = sel;
+      if (lane == 0) {
+        cbR[KB] = kc;
+        cbR[KB + 1] = c;
+      }
+    }
+    __syncthreads();
+    const int kc = uniform(cbR[KB]);
+    ci = uniform(cbR[KB + 1]);
+    if (kc == 0) return -1;
+    for (int u = 0; u < kc; ++u) publish_col(uniform(cbS[u]), cbM + u * RCAP);
+    __syncthreads();
+
+    // A: wavefront 0, the block's columns in registers (lane rows lane + 64 q)
+    if (wv == 0) {
+      T mv[QR][KB];
+      bool el[QR];
+#pragma unroll
+      for (int q = 0; q < QR; ++q) {
+        const int i = lane + 64 * q;
+        const int x = (i >= 2 && i < R) ? bs[i] : -1;
+        el[q] = x >= 0 && !redund[i] && !target[x];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) mv[q][u] = (u < kc && i < RCAP) ? cbM[u * RCAP + i] : T(0);
+      }
+#pragma unroll
+      for (int t = 0; t < KB; ++t) {
+        if (t < kc) {
+          T bv = inf_of<T>();
+          int bk = INT_MAX, bp = -1;
+          {
+            ArgMin<T> am;
+#pragma unroll
+            for (int q = 0; q < QR; ++q) am.add(el[q], -tabs(mv[q][t]), lane + 64 * q, lane + 64 * q);
+            wave_finish(am, bv, bk, bp);
+          }
+          const int r = uniform(bp);
+          int rr = -1;
+          if (r >= 0 && -bv > eps_p) {
+            rr = r;
+            const int rl = r & 63, rq = r >> 6;
+            T pv = mv[0][t];
+#pragma unroll
+            for (int q = 1; q < QR; ++q) pv = (rq == q) ? mv[q][t] : pv;
+            const T rp = T(1) / lane_read(pv, rl);
+#pragma unroll
+            for (int u = t + 1; u < KB; ++u) {
+              if (u < kc) {
+                T x = mv[0][u];
+#pragma unroll
+                for (int q = 1; q < QR; ++q) x = (rq == q) ? mv[q][u] : x;
+                const T pr = lane_read(x, rl) * rp;
+#pragma unroll
+                for (int q = 0; q < QR; ++q)
+                  mv[q][u] = (lane + 64 * q == r) ? pr : fma_t(-mv[q][t], pr, mv[q][u]);
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < QR; ++q) el[q] = el[q] && (lane + 64 * q != r);
+            if (lane == 0) cbRp[t] = rp;
+          }
+          if (lane == 0) cbR[t] = rr;
+        }
+      }
+      // column t as step t multiplied by (no later step of the block changes it)
+#pragma unroll
+      for (int q = 0; q < QR; ++q) {
+        const int i = lane + 64 * q;
+        if (i < RCAP) {
+#pragma unroll
+          for (int u = 0; u < KB; ++u)
+            if (u < kc) cbM[u * RCAP + i] = mv[q][u];
+        }
+      }
+      // basis changes: each step has its own row, column, entering and leaving variable,
+      // so lane u makes step u's
+      const int r = lane < kc ? cbR[lane] : -1;
+      const unsigned long long made = __ballot(r >= 0);
+      if (lane == 0) cbR[KB + 2] = __popcll(made);
+      if (r >= 0) {
+        {
+          const int s = cbS[lane];
+          const int lv = bs[r], e = nb[s];
+          const int kill = (lv >= n && lv < n + m && sense && sense[lv - n] == 1) ? 1 : 0;
+          cbW[__popcll(made & ((1ull << lane) - 1ull))] = r | (s << 12) | (lane << 24) | (kill << 30);
+          bs[r] = e;
+          nb[s] = lv;
+          colof[e] = -1;
+          colof[lv] = s;
+        }
+      }
+    }
+    __syncthreads();
+    for (int t = 0; t < kc; ++t) {
+      const int r = uniform(cbR[t]);
+      if (r >= 0) publish_row(r, cbP + t * CCAP);
+    }
+    __syncthreads();
+
+    // B: thread per column, the block's pivot rows in registers
+    for (int j = tid; j < C; j += NT) {
+      T pc[KB];
+#pragma unroll
+      for (int u = 0; u < KB; ++u) pc[u] = u < kc ? cbP[u * CCAP + j] : T(0);
+#pragma unroll
+      for (int t = 0; t < KB; ++t) {
+        if (t < kc) {
+          const int r = cbR[t];
+          if (r >= 0) {
+            const int s = cbS[t];
+            const T rp = cbRp[t];
+            const T pr = (j == s) ? rp : pc[t] * rp;
+            pc[t] = pr;
+#pragma unroll
+            for (int u = t + 1; u < KB; ++u) {
+              if (u < kc) {
+                const int ru = cbR[u];
+                if (ru >= 0) pc[u] = fma_t(-cbM[t * RCAP + ru], pr, (j == s) ? T(0) : pc[u]);
+              }
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (u < kc) cbP[u * CCAP + j] = pc[u];
+    }
+    __syncthreads();
+    return uniform(cbR[KB + 2]);
+  }
+
+
   __device__ __forceinline__ int nb_of(int j) {
     __syncthreads();
     return uniform(nb[j]);
@@ -522,7 +745,7 @@ struct TileTab {
 };
 
 // WPE: minimum waves per SIMD the register allocator must allow (1: no constraint)
-template <typename T, int NW, int TC, int RPT, int CPT, int WPE>
+template <typename T, int NW, int TC, int RPT, int CPT, int WPE, bool BC>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_tile(const LpDesc* __restrict__ desc,
                                                   const double* __restrict__ pool,
                                                   const int8_t* __restrict__ spool,
@@ -530,14 +753,16 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) 
                                                   const int32_t* count_dev, int32_t* queue,
                                                   Out out, Opt<T> o,
                                                   const int32_t* __restrict__ warm_pool) {
-  using Tab = TileTab<T, NW, TC, RPT, CPT>;
-  __shared__ T colbuf[Tab::RCAP], rhsbuf[Tab::RCAP], rowbuf[Tab::CCAP], scal[2];
+  using Tab = TileTab<T, NW, TC, RPT, CPT, BC>;
+  __shared__ T rhsbuf[Tab::RCAP], scal[2];
   __shared__ int bs[Tab::RCAP], nb[Tab::CCAP];
   __shared__ uint8_t redund[Tab::RCAP], target[Tab::RCAP + Tab::CCAP];
   __shared__ int wl[Tab::RCAP], colof[Tab::RCAP + Tab::CCAP];
   __shared__ Decision<T> slots[2];
   __shared__ double s_am[NW];
   __shared__ int s_q;
+  __shared__ T cbM[Tab::KB * Tab::RCAP], cbP[Tab::KB * Tab::CCAP], cbRp[Tab::KB];
+  __shared__ int cbR[Tab::KB + 3], cbS[Tab::KB], cbW[Tab::KB];
   Tab t;
   t.s_am = s_am;
   t.tid = threadIdx.x;
@@ -546,10 +771,12 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) 
   t.gr = threadIdx.x / TC;
   t.gc = threadIdx.x - t.gr * TC;
   t.phase = 0;
-  t.colbuf = colbuf; t.rhsbuf = rhsbuf; t.rowbuf = rowbuf; t.scal = scal;
+  t.colbuf = cbM; t.rhsbuf = rhsbuf; t.rowbuf = cbP; t.scal = scal;  // slot 0
   t.bs = bs; t.nb = nb; t.redund = redund; t.target = target;
   t.wl = wl; t.colof = colof;
   t.slots = slots;
+  t.cbM = cbM; t.cbP = cbP; t.cbRp = cbRp; t.cbR = cbR; t.cbS = cbS; t.cbW = cbW;
+  t.blk = -1;
   if (count_dev) count = *count_dev;
   for (;;) {
     __syncthreads();
@@ -574,22 +801,24 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) 
 // ---------------------------------------------------------------------------------
 // variant table: (waves per LP, TC, RPT, CPT); capacity (64*NW/TC*RPT) x (TC*CPT)
 // (the autotuner measures every variant that holds a shape; fewer waves per LP mean fewer
-// barriers per pivot, more waves mean fewer registers per thread).  The last two columns
-// are the waves-per-SIMD floors given to the register allocator for FP32 / FP64: set
-// where a few bytes of spill buy one more resident workgroup per CU.
-#define BLPG_TILE_VARIANTS(X)        \
-  X(0, 1, 8, 4, 5, 1, 1)             \
-  X(1, 1, 8, 7, 7, 4, 1)             \
-  X(2, 2, 16, 8, 7, 1, 1)            \
-  X(3, 2, 16, 10, 6, 1, 1)           \
-  X(4, 4, 16, 7, 7, 4, 1)            \
-  X(5, 8, 16, 6, 9, 1, 1)            \
-  X(6, 8, 16, 7, 13, 1, 1)           \
-  X(7, 1, 8, 10, 11, 1, 1)           \
-  X(8, 1, 16, 16, 7, 1, 1)           \
-  X(9, 2, 16, 14, 7, 1, 1)           \
-  X(10, 4, 32, 14, 4, 1, 1)          \
-  X(11, 8, 16, 4, 7, 1, 1)
+// barriers per pivot, more waves mean fewer registers per thread).  w32 / w64: the
+// waves-per-SIMD floors given to the register allocator for FP32 / FP64, set where a few
+// bytes of spill buy one more resident workgroup per CU.  b32 / b64: block crash on (the
+// variants whose register block leaves room for it without spilling).
+//   id nw  tc rpt cpt w32 w64 b32 b64
+#define BLPG_TILE_VARIANTS(X)              \
+  X(0, 1, 8, 4, 5, 1, 1, 1, 1)             \
+  X(1, 1, 8, 7, 7, 4, 1, 1, 1)             \
+  X(2, 2, 16, 8, 7, 1, 1, 1, 1)            \
+  X(3, 2, 16, 10, 6, 1, 1, 1, 1)           \
+  X(4, 4, 16, 7, 7, 4, 1, 1, 1)            \
+  X(5, 8, 16, 6, 9, 1, 1, 1, 0)            \
+  X(6, 8, 16, 7, 13, 1, 1, 0, 0)           \
+  X(7, 1, 8, 10, 11, 1, 1, 0, 0)           \
+  X(8, 1, 16, 16, 7, 1, 1, 0, 0)           \
+  X(9, 2, 16, 14, 7, 1, 1, 0, 0)           \
+  X(10, 4, 32, 14, 4, 1, 1, 1, 1)          \
+  X(11, 8, 16, 4, 7, 1, 1, 1, 1)
 
 constexpr int kTileVariants = 12;
 constexpr int kTileDefault = 7;  // static routing uses variants 0..6
@@ -599,7 +828,7 @@ int tile_default_variants() { return kTileDefault; }
 
 TileVariant tile_variant(int, int v) {
   switch (v) {
-#define X(id, nw, tc, rpt, cpt, w32, w64) \
+#define X(id, nw, tc, rpt, cpt, w32, w64, b32, b64) \
   case id: return TileVariant{nw, (nw * 64) / tc, rpt, cpt};
     BLPG_TILE_VARIANTS(X)
 #undef X
@@ -607,29 +836,40 @@ TileVariant tile_variant(int, int v) {
   }
 }
 
+// warm: the batch carries warm-start lists (block crash where the variant has it)
 template <typename T>
-static const void* tile_fn(int v) {
+static const void* tile_fn(int v, bool warm) {
   switch (v) {
-#define X(id, nw, tc, rpt, cpt, w32, w64) \
-  case id: return reinterpret_cast<const void*>(&k_tile<T, nw, tc, rpt, cpt, sizeof(T) == 4 ? w32 : w64>);
+#define X(id, nw, tc, rpt, cpt, w32, w64, b32, b64)                                           \
+  case id:                                                                                     \
+    if (warm && (sizeof(T) == 4 ? b32 : b64) != 0)                                             \
+      return reinterpret_cast<const void*>(                                                    \
+          &k_tile<T, nw, tc, rpt, cpt, sizeof(T) == 4 ? w32 : w64, true>);                     \
+    return reinterpret_cast<const void*>(&k_tile<T, nw, tc, rpt, cpt, sizeof(T) == 4 ? w32 : w64, false>);
     BLPG_TILE_VARIANTS(X)
 #undef X
     default: return nullptr;
   }
 }
 
+// the smaller of the cold and warm instantiations' residency
 int tile_max_blocks_per_cu(int precision, int v) {
-  const void* fn = precision == 0 ? tile_fn<double>(v) : tile_fn<float>(v);
-  if (!fn) return 0;
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, tile_variant(precision, v).nw * 64,
-                                                   0) != hipSuccess)
-    return 0;
-  return nb;
+  int best = -1;
+  for (int warm = 0; warm < 2; ++warm) {
+    const void* fn = precision == 0 ? tile_fn<double>(v, warm) : tile_fn<float>(v, warm);
+    if (!fn) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, tile_variant(precision, v).nw * 64,
+                                                     0) != hipSuccess)
+      return 0;
+    best = best < 0 || nb < best ? nb : best;
+  }
+  return best;
 }
 
 hipError_t launch_tile(const TileLaunch& L, hipStream_t st) {
-  const void* fn = L.precision == 0 ? tile_fn<double>(L.variant) : tile_fn<float>(L.variant);
+  const bool warm = L.warm != nullptr;
+  const void* fn = L.precision == 0 ? tile_fn<double>(L.variant, warm) : tile_fn<float>(L.variant, warm);
   if (!fn) return hipErrorInvalidValue;
   const int nt = tile_variant(L.precision, L.variant).nw * 64;
   if (L.precision == 0) {

@@ -288,26 +288,52 @@ __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* 
   it = 0; p1 = 0; status = kStatusOptimal; unb = -1; wfin = T(0);
   int stage = warm ? kStCrash : (sense ? kStEq : kStAux);
   int ei = 0, deg = 0, ci = 0;
+  int bt = 0, bnp = 0;  // block crash: next pivot of the prepared block, its pivots
   for (;;) {
     t.loop_fence();
     int pr = -1, ps = -1;
     bool kill_after = false, count_p1 = false;
     if (stage == kStCrash) {
-      int v = -1;
-      while (ci < m && v < 0) {
-        const int w = t.warm_at(warm, ci);
-        ++ci;
-        if (w >= 0 && w < n) v = w;
+      bool blocked = false;
+      if constexpr (Tab::kBlockCrash) {
+        // up to KB targets prepared at once, then one pivot per turn from the prepared
+        // block through the shared pivot site: the same pivots, in the same order, as the
+        // one-at-a-time crash below.  Prepared only when none can reach the iteration limit.
+        if (bt < bnp) {
+          const int wd = t.blk_word(bt++);
+          pr = wd & 0xFFF;
+          ps = (wd >> 12) & 0xFFF;
+          t.blk = (wd >> 24) & 0x3F;
+          kill_after = ((wd >> 30) & 1) != 0;
+          count_p1 = true;
+          blocked = true;
+        } else if (it + Tab::KB <= maxit) {
+          bt = 0;
+          bnp = t.crash_prepare(ci, sense, o.eps_p);
+          if (bnp < 0) {
+            bnp = 0;
+            stage = sense ? kStEq : kStDualChk;
+          }
+          continue;
+        }
       }
-      if (v < 0) { stage = sense ? kStEq : kStDualChk; continue; }
-      const int s = t.find_nonbasic(v);
-      if (s < 0) continue;
-      if (it >= maxit) { status = kStatusIterLimit; break; }
-      const int r = t.absmax_row(s, o.eps_p);
-      if (r < 0) continue;
-      const int lv = t.bs_now(r);  // absmax_row ended at a barrier, no pivot since
-      pr = r; ps = s; count_p1 = true;
-      kill_after = lv >= n && lv < n + m && sense && sense[lv - n] == 1;
+      if (!blocked) {
+        int v = -1;
+        while (ci < m && v < 0) {
+          const int w = t.warm_at(warm, ci);
+          ++ci;
+          if (w >= 0 && w < n) v = w;
+        }
+        if (v < 0) { stage = sense ? kStEq : kStDualChk; continue; }
+        const int s = t.find_nonbasic(v);
+        if (s < 0) continue;
+        if (it >= maxit) { status = kStatusIterLimit; break; }
+        const int r = t.absmax_row(s, o.eps_p);
+        if (r < 0) continue;
+        const int lv = t.bs_now(r);  // absmax_row ended at a barrier, no pivot since
+        pr = r; ps = s; count_p1 = true;
+        kill_after = lv >= n && lv < n + m && sense && sense[lv - n] == 1;
+      }
     } else if (stage == kStEq) {
       while (ei < m && sense[ei] != 1) ++ei;
       if (ei >= m) { stage = warm ? kStDualChk : kStAux; continue; }

@@ -142,6 +142,66 @@ def test_warm_start_matches_oracle():
         S.close()
 
 
+def scrambled(basis, n, m, rng):
+    """A warm list the crash must read literally: shuffled, a few entries dropped (-1),
+    repeated, or swapped for other structurals / slacks / x0."""
+    w = rng.permutation(np.asarray(basis, dtype=np.int32))
+    for i in range(m):
+        u = rng.random()
+        if u < 0.08:
+            w[i] = -1
+        elif u < 0.16:
+            w[i] = w[rng.integers(0, m)]
+        elif u < 0.24:
+            w[i] = rng.integers(0, n)
+        elif u < 0.28:
+            w[i] = rng.integers(n, n + m + 1)
+    return w
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_block_crash_matches_oracle(precision):
+    # the tile kernel crashes up to a block of targets at once; every pivot must be the
+    # one-at-a-time crash's (same rows, same tableau bits).  Cap 30: the first block fits
+    # under the cap, the rest of the crash runs one target at a time.
+    from tests.test_verify_ref import mixed_sense_lps
+    rng = np.random.default_rng(17)
+    lps = sample(2, 6) + [sa.LP(A, b, c, s) for A, b, c, s in mixed_sense_lps(8, m=40, n=34, seed=9)]
+    cases = []
+    for lp in lps:
+        o64 = oracle.solve(lp.A, lp.b, lp.c, lp.sense)
+        bs = o64["basis"]
+        m, n = lp.A.shape
+        cases.append((lp, bs, None, o64))                        # the optimal basis
+        cases.append((lp, scrambled(bs, n, m, rng), None, o64))  # read as given
+        cases.append((lp, bs, 30, o64))                          # cap inside the crash
+    # several blocks: more basic structurals than any block holds
+    assert max(int(((c[1] >= 0) & (c[1] < c[0].A.shape[1])).sum()) for c in cases) > 16
+    for cap in (None, 30):
+        sel = [c for c in cases if c[2] == cap]
+        wl = [sa.LP(lp.A, lp.b, lp.c, lp.sense, warm_basis=w) for lp, w, _, _ in sel]
+        S = sa.Solver(precision=precision, kernel="tile", verify=False, max_iter=cap or 0)
+        r = S.solve(wl)
+        exact = 0
+        for k, (lp, w, _, o64) in enumerate(sel):
+            if r.fallback[k]:  # an FP32 result the FP64 check sent to an FP64 re-solve
+                continue
+            o = oracle.solve(lp.A, lp.b, lp.c, lp.sense, precision=precision, warm=w,
+                             max_iter=cap)
+            assert r.status[k] == o["status"], (k, cap)
+            assert r.pivots[k] == o["pivots"] and r.phase1_pivots[k] == o["phase1_pivots"], (k, cap)
+            assert np.array_equal(r.basis[k], o["basis"]), (k, cap)
+            if precision == "fp64":
+                assert r.objective[k] == o["objective"], (k, cap)  # tableau bits
+                assert np.array_equal(r.x[k], o["x"]), (k, cap)
+            elif o["status"] == oracle.OPTIMAL:
+                # FP32 reports the FP64 check's objective of the same basis
+                assert rel_err(r.objective[k], o64["objective"]) <= 1e-9, (k, cap)
+            exact += 1
+        assert exact >= len(sel) // 3, (cap, exact)
+        S.close()
+
+
 def test_shared_A_matches_individual():
     lp0 = sa.gen_config(1, 0, 27, 32, 1, batch_count=1024)[0]
     rng = np.random.default_rng(3)
