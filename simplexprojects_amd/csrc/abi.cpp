@@ -1000,6 +1000,13 @@ int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps
       if (stream_max_blocks_per_cu(prec, false, nt, m + 2, n + 2) > 0)
         cand.push_back({BLPG_KERNEL_STREAM, nt});
     }
+    // the static routing's tile variant is timed first and kept unless another instance is
+    // more than 3% faster: closer timings are run-to-run noise, and a choice that flips
+    // between runs moves shapes between bins (and the batch's critical path with them)
+    const int tv0 = Tuner::best_tile(m, n, prec);
+    std::stable_partition(cand.begin(), cand.end(), [&](const std::pair<int, int>& c) {
+      return c.first == BLPG_KERNEL_TILE && c.second == tv0;
+    });
     double best_ms = 1e30;
     std::pair<int, int> best{-1, -1};
     for (const auto& c : cand) {
@@ -1017,7 +1024,8 @@ int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps
       }
       blpg_batch_free(ctx, B);
       if (rc) break;
-      if (t < best_ms) { best_ms = t; best = c; }
+      const bool incumbent_static = best.first == BLPG_KERNEL_TILE && best.second == tv0;
+      if (t < best_ms * (incumbent_static ? 0.97 : 1.0)) { best_ms = t; best = c; }
     }
     if (rc) break;
     if (best.first >= 0) ctx->tuner.set_preference(prec, m, n, best.first, best.second, best_ms);

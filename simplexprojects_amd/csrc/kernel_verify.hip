@@ -30,7 +30,7 @@
 // once into LDS (row-major, odd row stride: row sweeps and column sweeps are both free of
 // bank conflicts), so the sweeps run from LDS.  Up to three tiers run in sequence on a
 // bin, each with a larger LDS budget per LP (fewer LPs per CU) and each taking the LPs the
-// previous one deferred: k_verify<128> with 32 KB, k_verify<256> with 78 KB, and
+// previous one deferred: k_verify<128> with 31 KB, k_verify<128> with 78 KB, and
 // k_verify<256> with 150 KB, where an LP that still does not fit runs its sweeps from
 // global memory (rows batched per wavefront).
 #include <hip/hip_runtime.h>
@@ -107,6 +107,28 @@ __device__ __forceinline__ double bmax(double v, double* red) {
 #pragma unroll
     for (int w = 1; w < NT / 64; ++w) r = fmax(r, red[w]);
     return r;
+  }
+}
+// two maxima in one reduction (red: 2 * NT/64 doubles)
+template <int NT>
+__device__ __forceinline__ void bmax2(double& a, double& b, double* red) {
+  a = wmax(a);
+  b = wmax(b);
+  __syncthreads();
+  if constexpr (NT != 64) {
+    if ((threadIdx.x & 63) == 0) {
+      red[threadIdx.x >> 6] = a;
+      red[NT / 64 + (threadIdx.x >> 6)] = b;
+    }
+    __syncthreads();
+    double ra = red[0], rb = red[NT / 64];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) {
+      ra = fmax(ra, red[w]);
+      rb = fmax(rb, red[NT / 64 + w]);
+    }
+    a = ra;
+    b = rb;
   }
 }
 template <int NT>
@@ -229,8 +251,7 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   double bm = 0, cm = 0;
   for (int i = tid; i < m; i += NT) bm = fmax(bm, fabs(b[i]));
   for (int j = tid; j < n; j += NT) cm = fmax(cm, fabs(c[j]));
-  bm = bmax<NT>(bm, red);
-  cm = bmax<NT>(cm, red);
+  bmax2<NT>(bm, cm, red);
   const double am = out.amax[lp];
   const double sq_tol = sqrt(tol);
 
@@ -415,18 +436,15 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
         emit(a, sum);
       }
     };
-    auto vmax = [&](const double* vb) {
-      double l = 0;
-      for (int i = tid; i < m; i += NT) l = fmax(l, fabs(vb[i]));
-      return bmax<NT>(l, red);
-    };
     // refinement: stops at the rounding floor, after kMaxSweeps, or when a sweep no
     // longer shrinks the residual (a basis too ill-conditioned for the tableau's inverse)
     auto refine = [&](double* vb, bool ray, int e, double scale0, double& res, double& vm) {
       double prev = DBL_MAX;
       for (int sweep = 0;; ++sweep) {
-        res = bmax<NT>(resid(vb, ray, e), red);
-        vm = vmax(vb);
+        res = resid(vb, ray, e);
+        vm = 0;
+        for (int i = tid; i < m; i += NT) vm = fmax(vm, fabs(vb[i]));
+        bmax2<NT>(res, vm, red);
         if (sweep == kMaxSweeps || res <= 1e-14 * (scale0 + am * vm) || res > 0.9 * prev) break;
         prev = res;
         correct(vb);
@@ -487,8 +505,10 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
           rl = fmax(rl, fabs(y[p]));
         }
       }
-      rres_d = bmax<NT>(rl, red);
-      ym = vmax(y);
+      rres_d = rl;
+      ym = 0;
+      for (int p = tid; p < m; p += NT) ym = fmax(ym, fabs(y[p]));
+      bmax2<NT>(rres_d, ym, red);
       if (sweep == kMaxSweeps || rres_d <= 1e-14 * (1.0 + cm + am * ym) || rres_d > 0.9 * prev) break;
       prev = rres_d;
       if (staged) {
@@ -519,32 +539,24 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
     for (int p = tid; p < m; p += NT) w[p] = sg(p) * y[p];
     __syncthreads();
     double dvv = 0;
-    for (int j0 = tid; j0 < n; j0 += 2 * NT) {
-      // columns j0 and j0 + NT together: 16 independent loads per step
-      const int j1 = min(j0 + NT, n - 1);
-      double s0 = 0, s1 = 0, t0 = 0, t1 = 0;
+    for (int j = tid; j < n; j += NT) {
+      // one column per thread, 16 independent row loads in flight per step
+      double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
       int p = 0;
-      for (; p + 8 <= m; p += 8) {
-        double av[8], bv[8];
+      for (; p + 16 <= m; p += 16) {
+        double av[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          av[u] = A[(int64_t)(p + u) * n + j0];
-          bv[u] = A[(int64_t)(p + u) * n + j1];
-        }
+        for (int u = 0; u < 16; ++u) av[u] = A[(int64_t)(p + u) * n + j];
 #pragma unroll
-        for (int u = 0; u < 8; u += 2) {
+        for (int u = 0; u < 16; u += 4) {
           s0 = fma(av[u], w[p + u], s0);
           s1 = fma(av[u + 1], w[p + u + 1], s1);
-          t0 = fma(bv[u], w[p + u], t0);
-          t1 = fma(bv[u + 1], w[p + u + 1], t1);
+          s2 = fma(av[u + 2], w[p + u + 2], s2);
+          s3 = fma(av[u + 3], w[p + u + 3], s3);
         }
       }
-      for (; p < m; ++p) {
-        s0 = fma(A[(int64_t)p * n + j0], w[p], s0);
-        t0 = fma(A[(int64_t)p * n + j1], w[p], t0);
-      }
-      if (rowof[j0] < 0) dvv = fmax(dvv, c[j0] - (s0 + s1));
-      if (j0 + NT < n && rowof[j1] < 0) dvv = fmax(dvv, c[j1] - (t0 + t1));
+      for (; p < m; ++p) s0 = fma(A[(int64_t)p * n + j], w[p], s0);
+      if (rowof[j] < 0) dvv = fmax(dvv, c[j] - ((s0 + s1) + (s2 + s3)));
     }
     for (int p = tid; p < m; p += NT)
       if (colp[p] >= 0 && !is_eq(p)) dvv = fmax(dvv, -y[p]);
@@ -626,8 +638,9 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
 // Persistent, LPs assigned round-robin.  Dynamic LDS: [vectors (if lds_scratch)][stage_cap
 // bytes].  With a defer_list, LPs whose staged blocks exceed stage_cap go to it (the next
 // tier takes them); NT < 256 always defers them.
+// NT = 128: at most 168 VGPRs, three waves per SIMD (five LPs per CU, as its LDS allows)
 template <int NT>
-__global__ __launch_bounds__(NT) void k_verify(const LpDesc* __restrict__ desc,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 3 : 1))) void k_verify(const LpDesc* __restrict__ desc,
                                                const double* __restrict__ pool,
                                                const int8_t* __restrict__ spool,
                                                const int32_t* __restrict__ idx, int32_t count,
@@ -640,7 +653,7 @@ __global__ __launch_bounds__(NT) void k_verify(const LpDesc* __restrict__ desc,
                                                int32_t* defer_count) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_q;
-  __shared__ double red[NT / 64];
+  __shared__ double red[2 * (NT / 64)];
   __shared__ int red_i[NT / 64];
   char* wsp = lds_scratch ? (char*)smem : ws + (int64_t)blockIdx.x * ws_stride;
   char* stg = (char*)smem + (lds_scratch ? vec_bytes : 0);
@@ -667,12 +680,14 @@ __global__ __launch_bounds__(NT) void k_verify(const LpDesc* __restrict__ desc,
 
 namespace {
 // Tiers of the check, run in sequence on a bin; each takes the LPs the previous one
-// deferred.  0: 128 threads and 32 KB per LP (5 LPs per CU), for LPs with few basic
-// structurals; 1: 256 threads, 78 KB (2 per CU); 2: 256 threads, 150 KB (1 per CU), which
+// deferred.  0: 128 threads and 31 KB per LP (5 LPs per CU: at 32 KB the fifth one is not
+// resident although the occupancy query counts it), for LPs with few basic
+// structurals; 1: 128 threads, 78 KB (2 per CU; 256 threads would need 256 VGPRs and leave
+// one); 2: 256 threads, 150 KB (1 per CU), which
 // defers nothing (global-memory sweeps for what does not fit, vectors in HBM when even
 // they exceed 48 KB).
-constexpr int kTierNT[3] = {128, 256, 256};
-constexpr size_t kTierLds[3] = {32 * 1024 - 64, 78 * 1024, 150 * 1024};
+constexpr int kTierNT[3] = {128, 128, 256};
+constexpr size_t kTierLds[3] = {31 * 1024, 78 * 1024, 150 * 1024};
 constexpr size_t kVecLdsMax = 48 * 1024;
 
 size_t vec_bytes(int mcap, int ncap, int nt) { return VLayout(mcap, ncap, nt).total; }
@@ -704,7 +719,7 @@ bool verify_tier_used(int tier, int mcap, int ncap, int tsz) {
   if (tier == 0)  // worth a first pass when LPs with a dozen basic structurals fit
     return vec_in_lds(mcap, ncap, kTierNT[0]) && sc >= (int64_t)stage_bytes(mcap, 12, tsz);
   if (tier == 1)  // worth it when a quarter of the largest blocks fit
-    return vec_in_lds(mcap, ncap, 256) && sc >= (int64_t)stage_bytes(mcap, kmax / 4, tsz);
+    return vec_in_lds(mcap, ncap, kTierNT[1]) && sc >= (int64_t)stage_bytes(mcap, kmax / 4, tsz);
   return true;
 }
 
