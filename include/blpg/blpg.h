@@ -165,7 +165,9 @@ int blpg_batch_info_get(blpg_ctx *ctx, const blpg_batch *batch, blpg_batch_info 
  * blpg_autotune times, for every LP shape in `lps`, each kernel instance that can hold it
  * (register-tile variants, LDS-resident and HBM-streamed tableaux at 64..512 threads) on
  * those LPs in the context's precision (best of `reps` solves after one warm-up), and
- * routes later batches of that shape and precision to the fastest.  The choice changes
+ * routes later batches of that shape and precision to the fastest; the default route's
+ * tile variant is kept unless another instance is more than 3% faster (closer timings are
+ * run-to-run noise, and a route that flips moves shapes between bins).  The choice changes
  * speed only: all instances compute bit-identical results.  blpg_tune_set pins a route by
  * hand (param: tile variant, or threads per workgroup); export / import carry the table as
  * text lines "precision m n kernel param ms". */
