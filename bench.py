@@ -190,10 +190,12 @@ def cpu_baseline(lps, threads, min_wall=0.75):
     return r, dt, reps
 
 
-def serving_leg(sa, tune32=None, seconds=0.5, rate=50000.0):
+def serving_leg(sa, tune32=None, seconds=2.0, rate=50000.0):
     """Serving loop: Poisson arrivals at `rate` LPs/s of single LPs, shapes uniform over
     27x32, 50x48, 74x83, 105x103, 205x203 (generators as the first two workloads, seed 5),
-    FP32 tableau + FP64 check; latency counted from each scheduled arrival."""
+    FP32 tableau + FP64 check; latency counted from each scheduled arrival.  Two seconds of
+    arrivals: the drain after the last arrival (tens of ms, the largest shape's latency) is
+    a fixed cost that a short window would count as lost throughput."""
     shapes = [(27, 32), (50, 48), (74, 83), (105, 103), (205, 203)]
     pools = [sa.gen_config(5, 5, m, n, 512, shape_idx=i, batch_count=1024)
              for i, (m, n) in enumerate(shapes)]
