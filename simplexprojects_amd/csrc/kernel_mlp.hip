@@ -265,11 +265,22 @@ hipError_t launch_to_bf16(const float* p, uint16_t* pb, int64_t n, hipStream_t s
 // in decreasing order: warm[rank] = variable id.  One wavefront per row.
 // With `desc` the list of row r goes to the LP's basis slot of the batch; without it, to
 // warm_pool + row_lp[r] * m0 (LPs of shape m0 x n0).
+// Each logit becomes one 64-bit key ordered as (value, then lower id first): the float's
+// bits mapped to an increasing unsigned word (-0 taken as +0), then the inverted id.  The
+// rank of v is the number of keys above its own, one 64-bit compare per pair, the row's
+// keys read two per LDS load and held four per lane (rows of up to 256 variables; longer
+// rows take each lane's variables 64 apart in turn).
+__device__ __forceinline__ unsigned long long pick_key(float z, int v) {
+  const uint32_t u = __float_as_uint(z + 0.0f);
+  const uint32_t o = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return ((unsigned long long)o << 32) | (uint32_t)~(uint32_t)v;
+}
+
 __global__ __launch_bounds__(64) void k_pick_basis(const float* __restrict__ Z, int d_out_pad,
                                                    const int32_t* __restrict__ row_lp, int64_t rows,
                                                    const LpDesc* __restrict__ desc, int m0, int n0,
                                                    int32_t* __restrict__ warm_pool) {
-  extern __shared__ float zs[];
+  extern __shared__ __align__(16) unsigned long long zk[];
   const int64_t r = blockIdx.x;
   if (r >= rows) return;
   const int lp = row_lp[r];
@@ -281,17 +292,30 @@ __global__ __launch_bounds__(64) void k_pick_basis(const float* __restrict__ Z, 
     m = d.m; n = d.n; off = d.basis;
   }
   const int nv = n + m;
-  for (int j = threadIdx.x; j < nv; j += 64) zs[j] = Z[r * d_out_pad + j];
+  const int nv2 = (nv + 1) & ~1;  // key 0 pads to an even count: below every real key
+  for (int j = threadIdx.x; j < nv2; j += 64) zk[j] = j < nv ? pick_key(Z[r * d_out_pad + j], j) : 0ull;
   __syncthreads();
   int32_t* w = warm_pool + off;
-  for (int v = threadIdx.x; v < nv; v += 64) {
-    const float zv = zs[v];
-    int rank = 0;
-    for (int u = 0; u < nv; ++u) {
-      const float zu = zs[u];
-      rank += (zu > zv || (zu == zv && u < v)) ? 1 : 0;
+  const ulonglong2* z2 = reinterpret_cast<const ulonglong2*>(zk);
+  for (int v0 = threadIdx.x; v0 < nv; v0 += 256) {
+    unsigned long long kv[4];
+    int rk[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int v = v0 + 64 * q;
+      kv[q] = v < nv ? zk[v] : ~0ull;
+      rk[q] = 0;
     }
-    if (rank < m) w[rank] = v;
+    for (int u = 0; u < nv2; u += 2) {
+      const ulonglong2 t = z2[u >> 1];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rk[q] += (t.x > kv[q] ? 1 : 0) + (t.y > kv[q] ? 1 : 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int v = v0 + 64 * q;
+      if (v < nv && rk[q] < m) w[rk[q]] = v;
+    }
   }
 }
 
@@ -299,8 +323,9 @@ hipError_t launch_pick_basis(const float* Z, int d_out_pad, const int32_t* row_l
                              const LpDesc* desc, int m0, int n0, int32_t* warm_pool, int max_vars,
                              hipStream_t st) {
   if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pick_basis, dim3((unsigned)rows), dim3(64), sizeof(float) * max_vars, st, Z,
-                     d_out_pad, row_lp, rows, desc, m0, n0, warm_pool);
+  const size_t lds = sizeof(unsigned long long) * (size_t)((max_vars + 1) & ~1);
+  hipLaunchKernelGGL(k_pick_basis, dim3((unsigned)rows), dim3(64), lds, st, Z, d_out_pad, row_lp,
+                     rows, desc, m0, n0, warm_pool);
   return hipGetLastError();
 }
 
