@@ -280,7 +280,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   const size_t o_x = off; off += al(sizeof(double) * B->x_elems);
   const size_t o_basis = off; off += al(sizeof(int32_t) * B->basis_elems);
   const size_t o_tab = off; off += al(8 * (size_t)B->tab_elems);
-  const size_t o_cnt = off; off += al(sizeof(int32_t) * 128);
+  const size_t o_cnt = off; off += al(sizeof(int32_t) * 256);
   bool any_warm = false;
   for (int64_t k = 0; k < count; ++k) any_warm |= lps[k].warm_basis != nullptr;
   const size_t o_warm = off;
@@ -592,7 +592,7 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
 
   if (timing) HIPCHK(ctx, hipEventRecord(B->ev[0], st));
   HIPCHK(ctx, launch_fill_i32(B->d_fb, 0, B->count, st));
-  HIPCHK(ctx, launch_fill_i32(q, 0, 128, st));
+  HIPCHK(ctx, launch_fill_i32(q, 0, 256, st));
   // the solve kernels dump their final tableaux for the FP64 check
   B->out.tab = do_verify ? B->d_tab : nullptr;
   // learned warm start: crash lists of the family LPs, before any solve kernel
@@ -629,9 +629,9 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
     const Bin& b = B->bins[i];
     const int32_t cnt = (int32_t)b.ids.size();
     const int64_t pos = first[i];
-    int32_t* qb = q + 8 * i;  // [0] solve queue [1] verify queue [2] fail count
-                              // [3] fallback queue [4] re-verify queue
-                              // [5] deferred-check count [6] its queue
+    int32_t* qb = q + 16 * i;  // [0] solve queue [2] fail count [3] fallback queue
+                               // [4] re-check queue [5] [6] deferred-check counts
+                               // [8 + u] queue of check tier u
     hipStream_t s = lane_of[i] == 0 ? st : ctx->aux[lane_of[i] - 1];
     if (s != st && !used[lane_of[i]]) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->fork, 0));
     used[lane_of[i]] = true;
@@ -661,7 +661,7 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
         VerifyLaunch T = V;
         T.tier = tv[2 * u];
         T.grid = tv[2 * u + 1];
-        T.queue = qb + 1;
+        T.queue = qb + 8 + u;
         if (u > 0) {
           T.idx = ((u - 1) % 2 == 0 ? B->d_defer : B->d_defer2) + pos;
           T.count = -1;

@@ -635,9 +635,11 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   return true;
 }
 
-// Persistent, LPs assigned round-robin.  Dynamic LDS: [vectors (if lds_scratch)][stage_cap
-// bytes].  With a defer_list, LPs whose staged blocks exceed stage_cap go to it (the next
-// tier takes them); NT < 256 always defers them.
+// Persistent over an atomic queue (one LP per take, `queue` zeroed before the launch): a
+// workgroup that draws slow LPs takes fewer, so the launch ends with the LPs, not with the
+// unluckiest fixed share.  Dynamic LDS: [vectors (if lds_scratch)][stage_cap bytes].  With
+// a defer_list, LPs whose staged blocks exceed stage_cap go to it (the next tier takes
+// them); NT < 256 always defers them.
 // NT = 128: at most 168 VGPRs, three waves per SIMD (five LPs per CU, as its LDS allows)
 template <int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 3 : 1))) void k_verify(const LpDesc* __restrict__ desc,
@@ -659,12 +661,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 
   char* stg = (char*)smem + (lds_scratch ? vec_bytes : 0);
   const bool defer = defer_list != nullptr;
   if (count_dev) count = *count_dev;
-  // static round-robin over the LPs: the per-LP work is short and even, and one shared
-  // queue counter would serialise tens of thousands of atomics
-  (void)queue;
-  (void)s_q;
-  for (int q = blockIdx.x; q < count; q += gridDim.x) {
+  for (;;) {
     __syncthreads();
+    if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
+    __syncthreads();
+    const int q = s_q;
+    if (q >= count) break;
     const int lp = idx ? idx[q] : q;
     const LpDesc D = desc[lp];
     // the dump's precision: FP32 tableau, or FP64 for FP64 solves and FP64 fallbacks
@@ -748,7 +750,7 @@ size_t verify_ws_stride(int mcap, int ncap) {
 }
 
 hipError_t launch_verify(const VerifyLaunch& V, hipStream_t st) {
-  if (V.tier < 0 || V.tier > 2) return hipErrorInvalidValue;
+  if (V.tier < 0 || V.tier > 2 || !V.queue) return hipErrorInvalidValue;
   const int nt = kTierNT[V.tier];
   const bool defers = V.tier < 2 && V.defer_list;
   if (V.tier < 2 && !vec_in_lds(V.mcap, V.ncap, nt)) return hipErrorInvalidValue;
