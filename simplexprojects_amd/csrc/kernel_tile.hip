@@ -518,22 +518,24 @@ struct TileTab {
   // a pivot writes the map before its own barrier, and only pivots write it.
   __device__ __forceinline__ int find_nonbasic(int var) { return uniform(colof[var]); }
 
+  // pivot j of the prepared block, packed (wave-uniform)
+  __device__ __forceinline__ int blk_word(int j) const { return uniform(cbW[j]); }
+
   // Prepares the next up-to-KB crash targets as one block.  Same targets, rows and
   // arithmetic as one absmax_row + pivot per target, so the tableau ends bit-identical:
   //   targets  the warm-list structurals from ci on that are nonbasic, stopping before a
   //            repeated column (its second turn must see the first one's pivot);
   //   A        wavefront 0 runs the pivots on the targets' columns alone (cbM): step t
-  //            picks its row from column t as absmax_row does, makes the step's basis
-  //            change, and updates the later columns; column t keeps the values step t
-  //            multiplies by;
+  //            picks its row from column t as absmax_row does (rows pivoted by earlier
+  //            steps no longer eligible) and updates the later columns; column t keeps
+  //            the values step t multiplies by.  The steps' basis changes touch distinct
+  //            rows, columns and variables, so lane u then makes step u's;
   //   B        one thread per column j runs the pivot rows alone (cbP): row r_t, updated
   //            by steps < t, becomes step t's scaled pivot row.
   // The caller then makes the block's pivots (cbW) through pivot() with `blk` set to the
-  // step's slot, which reads the column and row from cbM / cbP and needs no barrier.  Five
+  // step's slot, which reads the column and row from cbM / cbP and needs no barrier.  Six
   // barriers per block instead of about five per target.  Returns the number of pivots
   // (steps without an eligible row make none), or -1 when no target is left (ci = m).
-  __device__ __forceinline__ int blk_word(int j) const { return uniform(cbW[j]); }
-
   __device__ int crash_prepare(int& ci, const int8_t* sense, T eps_p) {
     __syncthreads();
     // targets: wavefront 0 reads the list 64 entries at a time and takes the nonbasic
