@@ -272,30 +272,56 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
     for (int a = tid; a < k; a += NT) cS[a] = c[S[a]];
     if (staged && k > 0) {
       // one pass of gathers; thread (tp, ta) copies rows tp, tp + per, ... of column ta,
-      // eight rows per step (all loads of a step in flight together)
+      // eight rows per step with all loads of a step in flight together.  128 threads: the
+      // last step's rows past m re-read row m - 1 and are not stored; 256 threads (whose
+      // registers the clamped step would spill): the rows left after the full steps one
+      // at a time.
       const int per = k >= NT ? 1 : NT / k;
       const int ta = k >= NT ? tid : tid % k, tp = k >= NT ? 0 : tid / k;
       if (tp < per) {
         for (int a = ta; a < k; a += NT) {
           const int ca = S[a], cp = Pcol[a];
-          int p = tp;
-          for (; p + 7 * per < m; p += 8 * per) {
-            double av[8];
-            TT tv[8];
+          if constexpr (NT < 256) {
+            for (int p = tp; p < m; p += 8 * per) {
+              double av[8];
+              TT tv[8];
+              int8_t sv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              av[u] = A[(int64_t)(p + u * per) * n + ca];
-              tv[u] = T[(int64_t)(2 + p + u * per) * C + cp];
-            }
+              for (int u = 0; u < 8; ++u) {
+                const int pu = min(p + u * per, m - 1);
+                av[u] = A[(int64_t)pu * n + ca];
+                tv[u] = T[(int64_t)(2 + pu) * C + cp];
+                sv[u] = sense ? sense[pu] : (int8_t)0;
+              }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              AB[(p + u * per) * kpad + a] = sg(p + u * per) * av[u];
-              TB[(p + u * per) * kpad + a] = tv[u];
+              for (int u = 0; u < 8; ++u) {
+                const int pu = p + u * per;
+                if (pu < m) {
+                  AB[pu * kpad + a] = (sv[u] == 2 ? -1.0 : 1.0) * av[u];
+                  TB[pu * kpad + a] = tv[u];
+                }
+              }
             }
-          }
-          for (; p < m; p += per) {
-            AB[p * kpad + a] = sg(p) * A[(int64_t)p * n + ca];
-            TB[p * kpad + a] = T[(int64_t)(2 + p) * C + cp];
+          } else {
+            int p = tp;
+            for (; p + 7 * per < m; p += 8 * per) {
+              double av[8];
+              TT tv[8];
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                av[u] = A[(int64_t)(p + u * per) * n + ca];
+                tv[u] = T[(int64_t)(2 + p + u * per) * C + cp];
+              }
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                AB[(p + u * per) * kpad + a] = sg(p + u * per) * av[u];
+                TB[(p + u * per) * kpad + a] = tv[u];
+              }
+            }
+            for (; p < m; p += per) {
+              AB[p * kpad + a] = sg(p) * A[(int64_t)p * n + ca];
+              TB[p * kpad + a] = T[(int64_t)(2 + p) * C + cp];
+            }
           }
         }
       }
