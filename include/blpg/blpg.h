@@ -127,6 +127,14 @@ int blpg_batch_solve(blpg_ctx *ctx, blpg_batch *batch); /* enqueue on the contex
 int blpg_batch_sync(blpg_ctx *ctx);
 int blpg_batch_download(blpg_ctx *ctx, blpg_batch *batch, blpg_result *out);
 void blpg_batch_free(blpg_ctx *ctx, blpg_batch *batch);
+/* New LPs of the same shapes (and senses / warm lists present where they were) into an
+ * uploaded batch: host work only, into the batch's pinned staging, like the reference's
+ * setConstratint building its pinned tableau (simplex.cu:463-550).  The next
+ * blpg_batch_solve starts with the host-to-device copy of the inputs, as ComputeLP does
+ * (simplex.cu:944-953): each size bin's inputs on its own stream just before its kernel,
+ * so later bins' copies overlap earlier bins' solves.  Two LPs may share a new array only
+ * where the uploaded batch shared one (the upload stores each distinct array once). */
+int blpg_batch_repack(blpg_ctx *ctx, blpg_batch *batch, const blpg_lp *lps, int64_t count);
 
 /* Bulk results of the last blpg_batch_solve without per-LP copies: waits for the solve,
  * copies the batch's results with one DMA into host memory the batch owns (pinned) and

@@ -103,6 +103,7 @@ ABI_SYMBOLS = [
     "blpg_default_options", "blpg_create", "blpg_destroy", "blpg_last_error",
     "blpg_set_options", "blpg_solve_batch", "blpg_solve_shared_A", "blpg_batch_upload",
     "blpg_batch_solve", "blpg_batch_sync", "blpg_batch_download", "blpg_batch_free",
+    "blpg_batch_repack",
     "blpg_batch_fetch", "blpg_autotune", "blpg_tune_set", "blpg_tune_export", "blpg_tune_import",
     "blpg_batch_info_get", "blpg_gen_config", "blpg_version", "blpg_ws_create",
     "blpg_ws_destroy", "blpg_ws_train", "blpg_ws_predict", "blpg_batch_set_warmstart",
@@ -139,6 +140,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
     lib.blpg_batch_sync.argtypes = [C.c_void_p]
     lib.blpg_batch_download.argtypes = [C.c_void_p, C.c_void_p, P(_Result)]
     lib.blpg_batch_free.argtypes = [C.c_void_p, C.c_void_p]
+    lib.blpg_batch_repack.argtypes = [C.c_void_p, C.c_void_p, P(_LP), C.c_int64]
     lib.blpg_batch_fetch.argtypes = [C.c_void_p, C.c_void_p, P(_BatchResults)]
     lib.blpg_autotune.argtypes = [C.c_void_p, P(_LP), C.c_int64, C.c_int32]
     lib.blpg_tune_set.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -301,6 +303,15 @@ class Batch:
 
     def solve(self) -> None:
         self.solver._check(self.solver.lib.blpg_batch_solve(self.solver.ctx, self.h))
+
+    def repack(self, lps: Sequence[LP]) -> None:
+        """New LPs of the same shapes into the batch's pinned staging (host work); the next
+        solve() copies them to the device before its kernels."""
+        lps = list(lps)
+        arr = (_LP * len(lps))(*[_lp_struct(lp) for lp in lps])
+        self.solver._check(self.solver.lib.blpg_batch_repack(self.solver.ctx, self.h, arr,
+                                                             len(lps)))
+        self.lps = lps
 
     def sync(self) -> None:
         self.solver._check(self.solver.lib.blpg_batch_sync(self.solver.ctx))

@@ -63,13 +63,40 @@ int persistent_grid(const blpg_ctx* ctx, int per_cu, int64_t count) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, count));
 }
 
+// Copies the LPs' A, b, c and senses into the batch's pinned staging at the offsets the
+// upload chose (every distinct array once).
+void stage_inputs(blpg_batch* B, const blpg_lp* lps) {
+  double* pool = (double*)(B->h_stage + B->o_pool);
+  int8_t* spool = (int8_t*)(B->h_stage + B->o_spool);
+  for (int64_t k = 0; k < B->count; ++k) {
+    const blpg_lp& L = lps[k];
+    const LpDesc& d = B->desc[k];
+    std::memcpy(pool + d.a, L.A, sizeof(double) * (size_t)L.m * L.n);
+    std::memcpy(pool + d.b, L.b, sizeof(double) * L.m);
+    std::memcpy(pool + d.c, L.c, sizeof(double) * L.n);
+    if (L.sense) std::memcpy(spool + d.s, L.sense, L.m);
+  }
+}
+
+// Warm-start lists of the LPs that carry one (-1 elsewhere and for invalid ids).
+hipError_t upload_warm(blpg_batch* B, const blpg_lp* lps) {
+  std::vector<int32_t> w(B->basis_elems, -1);
+  for (int64_t k = 0; k < B->count; ++k)
+    if (lps[k].warm_basis)
+      for (int i = 0; i < lps[k].m; ++i) {
+        const int32_t v = lps[k].warm_basis[i];
+        w[B->desc[k].basis + i] = (v >= 0 && v < lps[k].n + lps[k].m) ? v : -1;
+      }
+  return hipMemcpy(B->d_warm, w.data(), sizeof(int32_t) * w.size(), hipMemcpyHostToDevice);
+}
+
 }  // namespace
 
-static void drop_graph(blpg_batch* B) {
-  if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
-  if (B->graph) (void)hipGraphDestroy(B->graph);
-  B->gexec = nullptr;
-  B->graph = nullptr;
+static void drop_graph(blpg_batch* B, int g) {
+  if (B->gexec[g]) (void)hipGraphExecDestroy(B->gexec[g]);
+  if (B->graph[g]) (void)hipGraphDestroy(B->graph[g]);
+  B->gexec[g] = nullptr;
+  B->graph[g] = nullptr;
 }
 
 // ------------------------------------------------------------------------------------
@@ -148,20 +175,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->desc.resize(count);
   B->family.resize(count);
   for (int64_t k = 0; k < count; ++k) B->family[k] = lps ? lps[k].family : -1;
-  // pack: distinct A / b / c / sense pointers are stored once (shared-A batches)
-  std::map<const void*, int64_t> seen;
-  std::vector<std::pair<const double*, int64_t>> dsrc;  // (ptr, len) in pool order
-  std::vector<std::pair<const int8_t*, int64_t>> ssrc;
-  auto place = [&](const double* p, int64_t len) -> int64_t {
-    auto it = seen.find(p);
-    if (it != seen.end()) return it->second;
-    const int64_t off = B->pool_elems;
-    B->pool_elems += (len + 7) & ~7ll;
-    dsrc.push_back({p, len});
-    seen[p] = off;
-    return off;
-  };
-  std::map<const void*, int64_t> sseen;
+  // shapes, senses and output offsets; the input pools are laid out once the bins are known
   for (int64_t k = 0; k < count; ++k) {
     const blpg_lp& L = lps[k];
     if (L.m < 1 || L.n < 1 || !L.A || !L.b || !L.c) {
@@ -173,41 +187,15 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
       delete B;
       return fail(ctx, BLPG_E_INVALID, "LP %lld: m, n above 4096 not supported", (long long)k);
     }
-    LpDesc& d = B->desc[k];
-    d.m = L.m;
-    d.n = L.n;
-    if (!seen.count(L.A) && !finite_all(L.A, (int64_t)L.m * L.n)) {
-      delete B;
-      return fail(ctx, BLPG_E_INVALID, "LP %lld: A has NaN/Inf", (long long)k);
-    }
-    if (!seen.count(L.b) && !finite_all(L.b, L.m)) {
-      delete B;
-      return fail(ctx, BLPG_E_INVALID, "LP %lld: b has NaN/Inf", (long long)k);
-    }
-    if (!seen.count(L.c) && !finite_all(L.c, L.n)) {
-      delete B;
-      return fail(ctx, BLPG_E_INVALID, "LP %lld: c has NaN/Inf", (long long)k);
-    }
-    d.a = place(L.A, (int64_t)L.m * L.n);
-    d.b = place(L.b, L.m);
-    d.c = place(L.c, L.n);
-    if (L.sense) {
+    if (L.sense)
       for (int i = 0; i < L.m; ++i)
         if (L.sense[i] < 0 || L.sense[i] > 2) {
           delete B;
           return fail(ctx, BLPG_E_INVALID, "LP %lld: sense[%d] not in {0,1,2}", (long long)k, i);
         }
-      auto it = sseen.find(L.sense);
-      if (it != sseen.end()) d.s = it->second;
-      else {
-        d.s = B->spool_elems;
-        B->spool_elems += (L.m + 15) & ~15ll;
-        ssrc.push_back({L.sense, L.m});
-        sseen[L.sense] = d.s;
-      }
-    } else {
-      d.s = -1;
-    }
+    LpDesc& d = B->desc[k];
+    d.m = L.m;
+    d.n = L.n;
     d.x = B->x_elems;
     B->x_elems += L.n;
     d.basis = B->basis_elems;
@@ -260,6 +248,70 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   if (B->bins.size() > 16) {
     delete B;
     return fail(ctx, BLPG_E_INVALID, "batch needs more than 16 kernel instances");
+  }
+
+  // ---- input pools.  Every distinct input array, keyed by (pointer, length), is stored
+  //      once (shared-A batches and parametric families share A; two prefixes of one buffer
+  //      with different lengths are different arrays).  An array that only one bin reads
+  //      lives in that bin's contiguous range, so a solve can copy each bin's inputs to the
+  //      device on the bin's own stream just before its kernel; arrays several bins read
+  //      form a common range in front, copied first. ----
+  {
+    std::map<SrcKey, int> key_bin;  // the one bin reading the array, or -1 (several)
+    for (int bi = 0; bi < (int)B->bins.size(); ++bi)
+      for (int32_t k : B->bins[bi].ids) {
+        const blpg_lp& L = lps[k];
+        for (const SrcKey& key : {SrcKey(L.A, (int64_t)L.m * L.n), SrcKey(L.b, L.m), SrcKey(L.c, L.n)}) {
+          auto it = key_bin.find(key);
+          if (it == key_bin.end()) key_bin[key] = bi;
+          else if (it->second != bi) it->second = -1;
+        }
+      }
+    int64_t bad = -1;
+    auto place = [&](const double* p, int64_t len, int64_t k) {
+      const SrcKey key(p, len);
+      if (B->pool_off.count(key)) return;
+      if (bad < 0 && !finite_all(p, len)) bad = k;
+      B->pool_off[key] = B->pool_elems;
+      B->pool_elems += (len + 7) & ~7ll;
+    };
+    auto place_lp = [&](int64_t k, int want_bin) {
+      const blpg_lp& L = lps[k];
+      if (key_bin[SrcKey(L.A, (int64_t)L.m * L.n)] == want_bin) place(L.A, (int64_t)L.m * L.n, k);
+      if (key_bin[SrcKey(L.b, L.m)] == want_bin) place(L.b, L.m, k);
+      if (key_bin[SrcKey(L.c, L.n)] == want_bin) place(L.c, L.n, k);
+    };
+    for (int64_t k = 0; k < count; ++k) place_lp(k, -1);
+    B->common_elems = B->pool_elems;
+    for (int bi = 0; bi < (int)B->bins.size(); ++bi) {
+      Bin& b = B->bins[bi];
+      b.pool_lo = B->pool_elems;
+      for (int32_t k : b.ids) place_lp(k, bi);
+      b.pool_hi = B->pool_elems;
+    }
+    if (bad >= 0) {
+      delete B;
+      return fail(ctx, BLPG_E_INVALID, "LP %lld: A, b or c has NaN/Inf", (long long)bad);
+    }
+    for (int64_t k = 0; k < count; ++k) {
+      const blpg_lp& L = lps[k];
+      LpDesc& d = B->desc[k];
+      d.a = B->pool_off[SrcKey(L.A, (int64_t)L.m * L.n)];
+      d.b = B->pool_off[SrcKey(L.b, L.m)];
+      d.c = B->pool_off[SrcKey(L.c, L.n)];
+      if (L.sense) {
+        const SrcKey key(L.sense, L.m);
+        auto it = B->spool_off.find(key);
+        if (it != B->spool_off.end()) d.s = it->second;
+        else {
+          d.s = B->spool_elems;
+          B->spool_elems += (L.m + 15) & ~15ll;
+          B->spool_off[key] = d.s;
+        }
+      } else {
+        d.s = -1;
+      }
+    }
   }
 
   // ---- one device allocation, carved ----
@@ -331,23 +383,19 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->d_counters = (int32_t*)(base + o_cnt);
   if (any_warm) B->d_warm = (int32_t*)(base + o_warm);
 
-  // ---- host staging (pinned) and one upload ----
-  const size_t stage_bytes = o_i32 + sizeof(int32_t) * n * 9;
-  char* h = nullptr;
-  if (hipHostMalloc((void**)&h, stage_bytes, hipHostMallocDefault) != hipSuccess) {
+  // ---- host staging (pinned, kept with the batch for repacks) and one upload ----
+  B->o_pool = o_pool;
+  B->o_spool = o_spool;
+  B->stage_bytes = o_i32 + sizeof(int32_t) * n * 9;
+  if (hipHostMalloc((void**)&B->h_stage, B->stage_bytes, hipHostMallocDefault) != hipSuccess) {
+    B->h_stage = nullptr;
     (void)hipFree(B->mem);
     delete B;
-    return fail(ctx, BLPG_E_NOMEM, "hipHostMalloc(%zu) failed", stage_bytes);
+    return fail(ctx, BLPG_E_NOMEM, "hipHostMalloc(%zu) failed", B->stage_bytes);
   }
+  char* h = B->h_stage;
   std::memcpy(h + o_desc, B->desc.data(), sizeof(LpDesc) * n);
-  for (auto& s : dsrc) {
-    const int64_t o = seen[s.first];
-    std::memcpy(h + o_pool + sizeof(double) * o, s.first, sizeof(double) * s.second);
-  }
-  for (auto& s : ssrc) {
-    const int64_t o = sseen[s.first];
-    std::memcpy(h + o_spool + o, s.first, (size_t)s.second);
-  }
+  stage_inputs(B, lps);
   int32_t* hi = (int32_t*)(h + o_i32);
   std::memset(hi, 0, sizeof(int32_t) * n * 8);
   for (int64_t k = 0; k < n; ++k) hi[6 * n + k] = (int32_t)k;
@@ -357,20 +405,11 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
     std::memcpy(hi + 8 * n + pos, b.ids.data(), sizeof(int32_t) * b.ids.size());
     pos += (int64_t)b.ids.size();
   }
-  hipError_t e = hipMemcpyAsync(B->mem, h, stage_bytes, hipMemcpyHostToDevice, ctx->stream);
+  hipError_t e = hipMemcpyAsync(B->mem, h, B->stage_bytes, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  (void)hipHostFree(h);
-  if (e == hipSuccess && any_warm) {
-    std::vector<int32_t> w(B->basis_elems, -1);
-    for (int64_t k = 0; k < count; ++k)
-      if (lps[k].warm_basis)
-        for (int i = 0; i < lps[k].m; ++i) {
-          const int32_t v = lps[k].warm_basis[i];
-          w[B->desc[k].basis + i] = (v >= 0 && v < lps[k].n + lps[k].m) ? v : -1;
-        }
-    e = hipMemcpy(B->d_warm, w.data(), sizeof(int32_t) * w.size(), hipMemcpyHostToDevice);
-  }
+  if (e == hipSuccess && any_warm) e = upload_warm(B, lps);
   if (e != hipSuccess) {
+    (void)hipHostFree(B->h_stage);
     (void)hipFree(B->mem);
     delete B;
     return fail(ctx, BLPG_E_HIP, "batch upload failed: %s", hipGetErrorString(e));
@@ -381,7 +420,8 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
       return fail(ctx, BLPG_E_HIP, "hipEventCreate failed");
     }
   for (int i = 0; i < 16; ++i)
-    if (hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&B->copied[i], hipEventDisableTiming) != hipSuccess) {
       blpg_batch_free(ctx, B);
       return fail(ctx, BLPG_E_HIP, "hipEventCreate failed");
     }
@@ -401,11 +441,15 @@ void blpg_batch_free(blpg_ctx* ctx, blpg_batch* B) {
   if (!B) return;
   if (ctx) (void)hipSetDevice(ctx->device);
   if (ctx) (void)hipStreamSynchronize(ctx->stream);
-  drop_graph(B);
+  drop_graph(B, 0);
+  drop_graph(B, 1);
   for (int i = 0; i < 96; ++i)
     if (B->ev[i]) (void)hipEventDestroy(B->ev[i]);
-  for (int i = 0; i < 16; ++i)
+  for (int i = 0; i < 16; ++i) {
     if (B->join[i]) (void)hipEventDestroy(B->join[i]);
+    if (B->copied[i]) (void)hipEventDestroy(B->copied[i]);
+  }
+  if (B->h_stage) (void)hipHostFree(B->h_stage);
   ws_free_rows(B->ws);
   if (B->h_res) (void)hipHostFree(B->h_res);
   if (B->warm_mem) (void)hipFree(B->warm_mem);
@@ -567,7 +611,7 @@ static constexpr int kEvWsStart = 66, kEvWsEnd = 67;  // learned warm start infe
 // to).  With `timing` every stage is bracketed by timing events; without, only the
 // fork / join events are recorded, so the sequence can be captured into a hipGraph.
 // Workspaces are grown first: nothing in the enqueued sequence allocates.
-static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
+static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing, bool copy) {
   hipStream_t st = ctx->stream;
   const int precision = ctx->opt.precision;
   B->precision = precision;
@@ -595,6 +639,26 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
   HIPCHK(ctx, launch_fill_i32(q, 0, 256, st));
   // the solve kernels dump their final tableaux for the FP64 check
   B->out.tab = do_verify ? B->d_tab : nullptr;
+  // inputs from the pinned staging (after a repack): the arrays several bins read and the
+  // senses first; each bin's own range on its stream just before its kernel (below), in
+  // launch order, one copy after the other -- unless the warm-start network, which reads
+  // every family LP's b and c, runs first
+  double* hpool = copy ? (double*)(B->h_stage + B->o_pool) : nullptr;
+  auto copy_range = [&](int64_t lo, int64_t hi, hipStream_t cs) -> hipError_t {
+    if (hi <= lo) return hipSuccess;
+    return hipMemcpyAsync(B->d_pool + lo, hpool + lo, sizeof(double) * (hi - lo),
+                          hipMemcpyHostToDevice, cs);
+  };
+  if (copy) {
+    HIPCHK(ctx, copy_range(0, B->common_elems, st));
+    if (B->spool_elems > 0)
+      HIPCHK(ctx, hipMemcpyAsync(B->d_spool, B->h_stage + B->o_spool, (size_t)B->spool_elems,
+                                 hipMemcpyHostToDevice, st));
+    if (B->model)
+      for (const Bin& b : B->bins) HIPCHK(ctx, copy_range(b.pool_lo, b.pool_hi, st));
+  }
+  const bool bin_copies = copy && !B->model;
+  int prev_copy = -1;
   // learned warm start: crash lists of the family LPs, before any solve kernel
   if (timing) HIPCHK(ctx, hipEventRecord(B->ev[kEvWsStart], st));
   if (B->model) {
@@ -635,6 +699,12 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing) {
     hipStream_t s = lane_of[i] == 0 ? st : ctx->aux[lane_of[i] - 1];
     if (s != st && !used[lane_of[i]]) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->fork, 0));
     used[lane_of[i]] = true;
+    if (bin_copies) {
+      if (prev_copy >= 0) HIPCHK(ctx, hipStreamWaitEvent(s, B->copied[prev_copy], 0));
+      HIPCHK(ctx, copy_range(b.pool_lo, b.pool_hi, s));
+      HIPCHK(ctx, hipEventRecord(B->copied[i], s));
+      prev_copy = i;
+    }
     if (timing) HIPCHK(ctx, hipEventRecord(B->ev[1 + 4 * i], s));
     // arena batches (serving) take their LP count from the device, so one captured graph
     // serves every batch size up to the capacity
@@ -712,38 +782,99 @@ int blpg_batch_solve(blpg_ctx* ctx, blpg_batch* B) {
   if (!ctx || !B) return BLPG_E_INVALID;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   if (B->count == 0) return BLPG_OK;
-  if (!ctx->opt.use_graph) return enqueue_solve(ctx, B, true);
-  // hipGraph path: the whole sequence (clears, warm-start network, every bin's solve /
-  // check / fallback on its stream, joins) is captured once per batch and replayed; it is
-  // captured again when options, the attached model or the context's workspaces change.
+  // after a repack the solve starts with the inputs' host-to-device copy (graph 1)
+  const int g = B->dirty ? 1 : 0;
+  if (!ctx->opt.use_graph) {
+    const int rc = enqueue_solve(ctx, B, true, g == 1);
+    if (!rc) B->dirty = false;
+    return rc;
+  }
+  // hipGraph path: the whole sequence (clears, input copies, warm-start network, every
+  // bin's solve / check / fallback on its stream, joins) is captured once per batch and
+  // variant and replayed; it is captured again when options, the attached model or the
+  // context's workspaces change.
   blpg_options key = ctx->opt;
   key.use_graph = 1;
-  const bool stale = !B->gexec || std::memcmp(&key, &B->g_opt, sizeof key) != 0 ||
-                     B->g_model != B->model || B->g_gen != ctx->gen;
+  const bool stale = !B->gexec[g] || std::memcmp(&key, &B->g_opt[g], sizeof key) != 0 ||
+                     B->g_model[g] != B->model || B->g_gen[g] != ctx->gen;
   if (stale) {
-    drop_graph(B);
+    drop_graph(B, g);
     int rc = plan_workspaces(ctx, B);  // allocations happen outside the capture
     if (rc) return rc;
     HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_solve(ctx, B, false);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess || !g) return fail(ctx, BLPG_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    B->graph = g;
-    if (hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
-      drop_graph(B);
+    rc = enqueue_solve(ctx, B, false, g == 1);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &gr);
+    if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+    if (e != hipSuccess || !gr) return fail(ctx, BLPG_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    B->graph[g] = gr;
+    if (hipGraphInstantiate(&B->gexec[g], gr, nullptr, nullptr, 0) != hipSuccess) {
+      drop_graph(B, g);
       return fail(ctx, BLPG_E_HIP, "hipGraphInstantiate failed");
     }
-    B->g_opt = key;
-    B->g_model = B->model;
-    B->g_gen = ctx->gen;
+    B->g_opt[g] = key;
+    B->g_model[g] = B->model;
+    B->g_gen[g] = ctx->gen;
   }
   HIPCHK(ctx, hipEventRecord(B->ev[0], ctx->stream));
-  HIPCHK(ctx, hipGraphLaunch(B->gexec, ctx->stream));
+  HIPCHK(ctx, hipGraphLaunch(B->gexec[g], ctx->stream));
   HIPCHK(ctx, hipEventRecord(B->ev[kEvEnd], ctx->stream));
   B->nev = kEvEnd + 1;
   B->timed = false;
+  B->dirty = false;
+  return BLPG_OK;
+}
+
+// New LPs of the same shapes into an uploaded batch's pinned staging (host work only, the
+// reference's setConstratint, simplex.cu:463-550); the next blpg_batch_solve copies them to
+// the device before its kernels.  Arrays the upload stored once must stay shared the same
+// way (two LPs may share a new array only where they shared one before).
+int blpg_batch_repack(blpg_ctx* ctx, blpg_batch* B, const blpg_lp* lps, int64_t count) {
+  if (!ctx || !B || count != B->count || (count > 0 && !lps))
+    return fail(ctx, BLPG_E_INVALID, "blpg_batch_repack: the batch holds %lld LPs",
+                B ? (long long)B->count : 0LL);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::map<int64_t, SrcKey> owner;  // pool offset -> the array staged there
+  std::map<int64_t, SrcKey> sowner;
+  for (int64_t k = 0; k < count; ++k) {
+    const blpg_lp& L = lps[k];
+    const LpDesc& d = B->desc[k];
+    if (L.m != d.m || L.n != d.n || !L.A || !L.b || !L.c || (L.sense != nullptr) != (d.s >= 0) ||
+        (L.warm_basis != nullptr && !B->d_warm))
+      return fail(ctx, BLPG_E_INVALID, "blpg_batch_repack: LP %lld differs in shape, senses or "
+                  "warm list from the uploaded batch", (long long)k);
+    const SrcKey ka(L.A, (int64_t)L.m * L.n), kb(L.b, L.m), kc(L.c, L.n);
+    for (const auto& ok : {std::make_pair(d.a, ka), std::make_pair(d.b, kb), std::make_pair(d.c, kc)}) {
+      auto it = owner.find(ok.first);
+      if (it == owner.end()) {
+        if (!finite_all((const double*)ok.second.first, ok.second.second))
+          return fail(ctx, BLPG_E_INVALID, "LP %lld: A, b or c has NaN/Inf", (long long)k);
+        owner[ok.first] = ok.second;
+      } else if (it->second != ok.second) {
+        return fail(ctx, BLPG_E_INVALID, "blpg_batch_repack: LP %lld shares an array the "
+                    "uploaded batch stored once with a different array", (long long)k);
+      }
+    }
+    if (L.sense) {
+      for (int i = 0; i < L.m; ++i)
+        if (L.sense[i] < 0 || L.sense[i] > 2)
+          return fail(ctx, BLPG_E_INVALID, "LP %lld: sense[%d] not in {0,1,2}", (long long)k, i);
+      const SrcKey ks(L.sense, L.m);
+      auto it = sowner.find(d.s);
+      if (it == sowner.end()) sowner[d.s] = ks;
+      else if (it->second != ks)
+        return fail(ctx, BLPG_E_INVALID, "blpg_batch_repack: LP %lld shares a sense array the "
+                    "uploaded batch stored once with a different one", (long long)k);
+    }
+  }
+  // the previous solve's copies read the staging: wait for them
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  stage_inputs(B, lps);
+  if (B->d_warm) {
+    const hipError_t e = upload_warm(B, lps);
+    if (e != hipSuccess) return fail(ctx, BLPG_E_HIP, "warm lists: %s", hipGetErrorString(e));
+  }
+  B->dirty = true;
   return BLPG_OK;
 }
 
@@ -1009,6 +1140,10 @@ int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps
     });
     double best_ms = 1e30;
     std::pair<int, int> best{-1, -1};
+    // the shape's route before the measurement, restored if a candidate fails
+    int pk = 0, pp = 0;
+    double pms = -1.0;
+    const bool had = ctx->tuner.get_preference(prec, m, n, pk, pp, pms);
     for (const auto& c : cand) {
       ctx->tuner.set_preference(prec, m, n, c.first, c.second);
       blpg_batch* B = nullptr;
@@ -1027,7 +1162,11 @@ int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps
       const bool incumbent_static = best.first == BLPG_KERNEL_TILE && best.second == tv0;
       if (t < best_ms * (incumbent_static ? 0.97 : 1.0)) { best_ms = t; best = c; }
     }
-    if (rc) break;
+    if (rc) {
+      if (had) ctx->tuner.set_preference(prec, m, n, pk, pp, pms);
+      else ctx->tuner.erase_preference(prec, m, n);
+      break;
+    }
     if (best.first >= 0) ctx->tuner.set_preference(prec, m, n, best.first, best.second, best_ms);
   }
   ctx->opt = saved;

@@ -5,7 +5,9 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/blpg/blpg.h"
@@ -37,7 +39,11 @@ struct Bin {
   int rcap = 0, ccap = 0;
   std::vector<int32_t> ids;
   int32_t* d_idx = nullptr;
+  int64_t pool_lo = 0, pool_hi = 0;  // the bin's own range of the input pool (elements)
 };
+
+// One input array of a batch: (host pointer, length).  Arrays are stored once per key.
+typedef std::pair<const void*, int64_t> SrcKey;
 
 struct DevBuf {
   void* p = nullptr;
@@ -69,6 +75,11 @@ struct blpg_batch {
   int precision = 0;
   std::vector<blpg::LpDesc> desc;
   int64_t pool_elems = 0, spool_elems = 0, x_elems = 0, basis_elems = 0, tab_elems = 0;
+  int64_t common_elems = 0;  // pool[0, common_elems): arrays read by several bins
+  std::map<blpg::SrcKey, int64_t> pool_off, spool_off;  // where each distinct array lives
+  char* h_stage = nullptr;   // pinned host image of desc | pool | sense pool | int32 init
+  size_t stage_bytes = 0, o_pool = 0, o_spool = 0;
+  bool dirty = false;        // staging newer than the device copy: the next solve copies it
   char* d_tab = nullptr;  // final-tableau dumps (read by the FP64 check)
   int mmax = 0, nmax = 0;
   void* mem = nullptr;
@@ -99,11 +110,13 @@ struct blpg_batch {
   hipEvent_t join[16] = {};  // per-bin join of an aux stream back into the context stream
   int nev = 0;
   bool timed = false;        // last solve recorded per-stage timing events
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  blpg_options g_opt{};      // options / model / workspace generation the graph holds
-  blpg_ws_model* g_model = nullptr;
-  int64_t g_gen = -1;
+  hipEvent_t copied[16] = {};  // per bin: its inputs are on the device (solves with a copy)
+  // captured solves: [0] inputs resident, [1] inputs copied from the staging first
+  hipGraph_t graph[2] = {};
+  hipGraphExec_t gexec[2] = {};
+  blpg_options g_opt[2] = {};  // options / model / workspace generation each graph holds
+  blpg_ws_model* g_model[2] = {};
+  int64_t g_gen[2] = {-1, -1};
 };
 
 

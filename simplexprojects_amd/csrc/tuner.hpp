@@ -49,6 +49,20 @@ class Tuner {
     ms_[key(precision, m, n)] = ms;
   }
   void clear_preferences() { pref_.clear(); ms_.clear(); }
+  // the stored preference of a shape (false: none)
+  bool get_preference(int precision, int m, int n, int& kernel, int& param, double& ms) const {
+    const auto it = pref_.find(key(precision, m, n));
+    if (it == pref_.end()) return false;
+    kernel = it->second.first;
+    param = it->second.second;
+    const auto jt = ms_.find(key(precision, m, n));
+    ms = jt == ms_.end() ? -1.0 : jt->second;
+    return true;
+  }
+  void erase_preference(int precision, int m, int n) {
+    pref_.erase(key(precision, m, n));
+    ms_.erase(key(precision, m, n));
+  }
   // "precision m n kernel param ms" lines
   std::string export_text() const {
     std::string out;

@@ -629,7 +629,7 @@ int blpg_batch_set_warmstart(blpg_ctx* ctx, blpg_batch* B, blpg_ws_model* M) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   free_rows(B->ws);
   B->model = nullptr;
-  B->g_gen = -1;  // a captured graph holds the old rows: capture again
+  B->g_gen[0] = B->g_gen[1] = -1;  // captured graphs hold the old rows: capture again
   if (!M) return BLPG_OK;
   std::vector<int32_t> fam(B->count, -1), row_lp, row_fam;
   for (int64_t k = 0; k < B->count; ++k) {

@@ -146,17 +146,70 @@ def test_fetch_matches_download(precision):
 
 
 def test_batch_beyond_budget_is_solved_in_parts():
-    # out-of-core batching: a batch over the HBM budget is split and solved part by part
+    # out-of-core batching: a batch over the HBM budget is split and solved part by part;
+    # every part's results are the oracle's, bit for bit
     lps = sa.gen_config(2, 2, 50, 48, 600, shape_idx=0, batch_count=1024)
-    S = sa.Solver(precision="fp64")
-    ref = S.solve(lps)
-    S.close()
+    ref = [oracle.solve(lp.A, lp.b, lp.c) for lp in lps]
     # ~42 KB of device memory per LP of this shape: 6 MB holds about a quarter of them
-    S2 = sa.Solver(precision="fp64", hbm_budget=6 << 20)
+    S2 = sa.Solver(precision="fp64", verify=False, hbm_budget=6 << 20)
     r = S2.solve(lps)
-    assert np.array_equal(r.status, ref.status)
-    assert np.array_equal(r.pivots, ref.pivots)
-    assert np.array_equal(r.objective, ref.objective)
+    for k in range(len(lps)):
+        assert r.status[k] == ref[k]["status"], k
+        assert r.pivots[k] == ref[k]["pivots"], k
+        assert r.objective[k] == ref[k]["objective"], k
+        assert np.array_equal(r.basis[k], ref[k]["basis"]), k
     with pytest.raises(sa.BlpgError):
         S2.upload(lps)  # one device batch of all of them still exceeds the budget
     S2.close()
+
+
+def test_prefix_arrays_of_one_buffer_are_distinct():
+    # two LPs whose b (and A) are prefixes of one buffer with different lengths: each is
+    # its own array in the device pool (the pool keys arrays by pointer and length)
+    rng = np.random.default_rng(11)
+    big_A = rng.uniform(-1, 1, size=(12 * 8,))
+    big_b = rng.uniform(1, 5, size=12)
+    c = rng.uniform(0, 1, size=8)
+    lps = []
+    for m in (6, 12, 9):
+        lp = sa.LP(np.zeros((m, 8)), np.zeros(m), c)
+        lp.A = big_A[: m * 8].reshape(m, 8)  # views: same base pointer, different lengths
+        lp.b = big_b[:m]
+        lps.append(lp)
+    assert lps[0].A.ctypes.data == lps[1].A.ctypes.data == lps[2].A.ctypes.data
+    S = sa.Solver(precision="fp64", verify=False)
+    r = S.solve(lps)
+    for k, lp in enumerate(lps):
+        o = oracle.solve(lp.A, lp.b, lp.c)
+        assert r.status[k] == o["status"] and r.pivots[k] == o["pivots"], k
+        assert r.objective[k] == o["objective"], k
+    S.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_repack_copies_new_inputs(precision):
+    # a repacked batch solves the new LPs (the copy to the device runs inside the solve,
+    # bin by bin), and a later solve without repack keeps them
+    a = sa.gen_config(2, 2, 50, 48, 48, shape_idx=0, batch_count=1024)
+    a += sa.gen_config(2, 2, 74, 83, 48, shape_idx=2, batch_count=1024)
+    b = sa.gen_config(2, 7, 50, 48, 48, shape_idx=0, batch_count=1024)
+    b += sa.gen_config(2, 7, 74, 83, 48, shape_idx=2, batch_count=1024)
+    S = sa.Solver(precision=precision)
+    batch = S.upload(a)
+    batch.solve()
+    ra = batch.fetch()
+    batch.repack(b)
+    batch.solve()
+    rb = batch.fetch()
+    batch.solve()
+    rb2 = batch.fetch()
+    direct = S.solve(b)
+    for r in (rb, rb2):
+        assert np.array_equal(r.status, direct.status)
+        assert np.array_equal(r.pivots, direct.pivots)
+        assert np.array_equal(r.objective, direct.objective)
+    assert not np.array_equal(ra.objective, rb.objective)
+    with pytest.raises(sa.BlpgError):
+        batch.repack(b[:10])  # a different LP count
+    batch.free()
+    S.close()
