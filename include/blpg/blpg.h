@@ -119,6 +119,25 @@ int blpg_solve_batch(blpg_ctx *ctx, const blpg_lp *lps, int64_t count, blpg_resu
 int blpg_solve_shared_A(blpg_ctx *ctx, int32_t m, int32_t n, const double *A, const double *b,
                         const int8_t *sense, const double *C, int64_t count, blpg_result *out);
 
+/* Shared-A batch solve that allocates nothing, for callers that cannot allocate (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same LPs and results as blpg_solve_shared_A with
+ * precision BLPG_FP64 and verify = 0 (status, objective = the tableau's, x, basis, pivots),
+ * on shapes the register-tile kernels hold.  Every pointer is a device pointer:
+ * A (m*n, row-major), b (m), sense (m, or NULL for all '<='), C (count*n); outputs status,
+ * objective (count), x (count*n), basis (count*m), pivots (count, or NULL).  The call
+ * enqueues on `stream` (a hipStream_t; NULL = the null stream) and returns; it does not
+ * synchronise.  Its descriptors, queue counter, staged A | b | C and internal outputs live
+ * in a static device array of 32 MiB per GPU (held until the process ends):
+ * blpg_noalloc_max_count(m, n) LPs at most, larger counts fail with BLPG_E_BUDGET.  Every
+ * call on a GPU shares that array, so calls that overlap on one GPU race: serialise them
+ * on one stream.  Errors: blpg_last_error(NULL), per host thread. */
+int blpg_solve_shared_A_noalloc(int32_t m, int32_t n, const double *A, const double *b,
+                                const int8_t *sense, const double *C, int64_t count,
+                                int32_t *status, double *objective, double *x, int32_t *basis,
+                                int32_t *pivots, void *stream);
+int64_t blpg_noalloc_max_count(int32_t m, int32_t n);
+
 /* Device-resident batches (inputs uploaded once, solved many times; used by the
  * serving loop and the benchmark).  No reference counterpart: the reference copies
  * the whole tableau host->device on every ComputeLP (simplex.cu:944-953). */

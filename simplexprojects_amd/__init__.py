@@ -109,7 +109,7 @@ ABI_SYMBOLS = [
     "blpg_ws_destroy", "blpg_ws_train", "blpg_ws_predict", "blpg_batch_set_warmstart",
     "blpg_ws_params", "blpg_ws_norm", "blpg_ws_gradient", "blpg_server_create",
     "blpg_server_destroy", "blpg_server_submit", "blpg_server_poll", "blpg_server_stats_get",
-    "blpg_server_replay",
+    "blpg_server_replay", "blpg_solve_shared_A_noalloc", "blpg_noalloc_max_count",
 ]
 
 _lib: Optional[C.CDLL] = None
@@ -152,6 +152,11 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
                                     C.c_int64, C.c_int32, C.c_int32, P(C.c_double),
                                     P(C.c_double), P(C.c_double), P(C.c_int8), P(C.c_int32)]
     lib.blpg_version.restype = C.c_char_p
+    V = C.c_void_p  # device pointers and the stream
+    lib.blpg_solve_shared_A_noalloc.argtypes = [C.c_int32, C.c_int32, V, V, V, V, C.c_int64,
+                                                V, V, V, V, V, V]
+    lib.blpg_noalloc_max_count.argtypes = [C.c_int32, C.c_int32]
+    lib.blpg_noalloc_max_count.restype = C.c_int64
     F, I32 = P(C.c_float), P(C.c_int32)
     lib.blpg_server_create.argtypes = [C.c_void_p, P(_ServerConfig), P(C.c_void_p)]
     lib.blpg_server_destroy.argtypes = [C.c_void_p]

@@ -34,7 +34,8 @@ HIPFLAGS = [
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROOT / 'include'}"]
 
 HIP_SOURCES = ["kernel_stream.hip", "kernel_tile.hip", "kernel_verify.hip", "kernel_util.hip",
-               "kernel_mlp.hip", "abi.cpp", "simplex_shim.cpp", "warmstart.cpp", "serve.cpp"]
+               "kernel_mlp.hip", "abi.cpp", "simplex_shim.cpp", "warmstart.cpp", "serve.cpp",
+               "noalloc.cpp"]
 SHIM_TEST = PKG / "test_shim"  # C++ drop-in test program (tests/cpp/test_shim.cpp)
 CXX_SOURCES = ["gen.cpp"]
 
@@ -51,7 +52,7 @@ def _run(cmd: list[str]) -> None:
 def _compile(src: Path) -> Path:
     obj = OBJ / (src.name + ".o")
     if src.suffix == ".hip" or src.name in ("abi.cpp", "simplex_shim.cpp", "warmstart.cpp",
-                                            "serve.cpp"):
+                                            "serve.cpp", "noalloc.cpp"):
         cmd = [HIPCC, *HIPFLAGS, "-x", "hip", "-c", str(src), "-o", str(obj)]
     else:
         cmd = ["g++", *CXXFLAGS, "-c", str(src), "-o", str(obj)]

@@ -92,6 +92,17 @@ hipError_t upload_warm(blpg_batch* B, const blpg_lp* lps) {
 
 }  // namespace
 
+// message of the last failed call that takes no context, per host thread
+static thread_local char g_free_err[512] = "no context";
+
+int blpg::set_free_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_free_err, sizeof g_free_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
 static void drop_graph(blpg_batch* B, int g) {
   if (B->gexec[g]) (void)hipGraphExecDestroy(B->gexec[g]);
   if (B->graph[g]) (void)hipGraphDestroy(B->graph[g]);
@@ -154,7 +165,7 @@ void blpg_destroy(blpg_ctx* ctx) {
   delete ctx;
 }
 
-const char* blpg_last_error(const blpg_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+const char* blpg_last_error(const blpg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_free_err; }
 
 int blpg_set_options(blpg_ctx* ctx, const blpg_options* opt) {
   if (!ctx || !opt) return BLPG_E_INVALID;

@@ -132,6 +132,10 @@ inline int fail(blpg_ctx* ctx, int code, const char* fmt, ...) {
   return code;
 }
 
+// error of a call that takes no context (the no-allocation entry point): kept per host
+// thread, returned by blpg_last_error(NULL); returns `code`
+int set_free_error(int code, const char* fmt, ...);
+
 // results of slots 0..k-1 of a batch (abi.cpp; the serving arenas use it)
 int batch_fetch_prefix(blpg_ctx* ctx, blpg_batch* B, int64_t k, blpg_batch_results* R);
 
