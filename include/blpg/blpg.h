@@ -48,7 +48,8 @@ extern "C" {
 #define BLPG_KERNEL_AUTO 0   /* autotuner picks per size bin */
 #define BLPG_KERNEL_STREAM 1 /* tableau streamed from HBM (any size) */
 #define BLPG_KERNEL_LDS 2    /* tableau resident in LDS (when it fits) */
-#define BLPG_KERNEL_TILE 3   /* tableau resident in VGPRs (when it fits) */
+#define BLPG_KERNEL_TILE 3   /* tableau resident in VGPRs, 2-D blocks per thread (when it fits) */
+#define BLPG_KERNEL_WAVE 4   /* tableau resident in VGPRs, whole columns per lane (when it fits) */
 
 typedef struct blpg_ctx blpg_ctx;
 typedef struct blpg_batch blpg_batch;

@@ -25,8 +25,8 @@ OK = 0
 E_INVALID, E_HIP, E_NOMEM, E_NODEVICE, E_BUDGET = -1, -2, -3, -4, -5
 UNDECIDED, INFEASIBLE, OPTIMAL, UNBOUNDED, ITER_LIMIT, NUMERIC = 1, 4, 5, 6, 100, 101
 FP64, FP32 = 0, 1
-KERNEL_AUTO, KERNEL_STREAM, KERNEL_LDS, KERNEL_TILE = 0, 1, 2, 3
-_KERNELS = {"auto": 0, "stream": 1, "lds": 2, "tile": 3}
+KERNEL_AUTO, KERNEL_STREAM, KERNEL_LDS, KERNEL_TILE, KERNEL_WAVE = 0, 1, 2, 3, 4
+_KERNELS = {"auto": 0, "stream": 1, "lds": 2, "tile": 3, "wave": 4}
 STATUS_NAMES = {1: "undecided", 4: "infeasible", 5: "optimal", 6: "unbounded",
                 100: "iter_limit", 101: "numeric"}
 

@@ -33,7 +33,7 @@ HIPFLAGS = [
 ]
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROOT / 'include'}"]
 
-HIP_SOURCES = ["kernel_stream.hip", "kernel_tile.hip", "kernel_verify.hip", "kernel_util.hip",
+HIP_SOURCES = ["kernel_wave.hip", "kernel_stream.hip", "kernel_tile.hip", "kernel_verify.hip", "kernel_util.hip",
                "kernel_mlp.hip", "abi.cpp", "simplex_shim.cpp", "warmstart.cpp", "serve.cpp",
                "noalloc.cpp"]
 SHIM_TEST = PKG / "test_shim"  # C++ drop-in test program (tests/cpp/test_shim.cpp)

@@ -225,6 +225,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
     routes[k] = ctx->tuner.route(B->desc[k].m, B->desc[k].n, ctx->opt.precision, ctx->opt.kernel);
   auto bin_key = [](const Route& r) -> long {
     if (r.kernel == BLPG_KERNEL_TILE) return 1000000L + r.variant;
+    if (r.kernel == BLPG_KERNEL_WAVE) return 4000000L + r.variant;
     if (r.kernel == BLPG_KERNEL_STREAM) return 2000000L + r.nt;
     return 3000000L + (long)r.nt * 100000L + r.bin;  // LDS: per size class
   };
@@ -442,7 +443,8 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
     B->info.rcap[i] = B->bins[i].rcap;
     B->info.ccap[i] = B->bins[i].ccap;
     B->info.count[i] = (int64_t)B->bins[i].ids.size();
-    B->info.param[i] = B->bins[i].kernel == BLPG_KERNEL_TILE ? B->bins[i].variant : B->bins[i].nt;
+    B->info.param[i] = (B->bins[i].kernel == BLPG_KERNEL_TILE || B->bins[i].kernel == BLPG_KERNEL_WAVE)
+                           ? B->bins[i].variant : B->bins[i].nt;
   }
   *out = B;
   return BLPG_OK;
@@ -476,6 +478,12 @@ struct BinPlan {
 };
 
 static int plan_bin(blpg_ctx* ctx, const Bin& b, int precision, int64_t count, BinPlan& P) {
+  if (b.kernel == BLPG_KERNEL_WAVE) {
+    const int per_cu = wave_max_blocks_per_cu(precision, b.variant);
+    if (per_cu <= 0) return fail(ctx, BLPG_E_INVALID, "wave variant %d cannot be resident", b.variant);
+    P.grid = persistent_grid(ctx, per_cu, count);
+    return BLPG_OK;
+  }
   if (b.kernel == BLPG_KERNEL_TILE) {
     const int per_cu = tile_max_blocks_per_cu(precision, b.variant);
     if (per_cu <= 0) return fail(ctx, BLPG_E_INVALID, "tile variant %d cannot be resident", b.variant);
@@ -505,7 +513,7 @@ static int launch_bin(blpg_ctx* ctx, blpg_batch* B, const Bin& b, const BinPlan&
   o.precision = precision;
   resolve(o, eps_d, eps_p, tol, kkt);
   HIPCHK(ctx, launch_fill_i32(queue, 0, 1, st));
-  if (b.kernel == BLPG_KERNEL_TILE) {
+  if (b.kernel == BLPG_KERNEL_TILE || b.kernel == BLPG_KERNEL_WAVE) {
     TileLaunch L{};
     L.precision = precision;
     L.variant = b.variant;
@@ -515,7 +523,8 @@ static int launch_bin(blpg_ctx* ctx, blpg_batch* B, const Bin& b, const BinPlan&
     L.max_iter = o.max_iter; L.bland_after = bland_after(o);
     L.grid = P.grid;
     L.warm = warm;
-    HIPCHK(ctx, launch_tile(L, st));
+    if (b.kernel == BLPG_KERNEL_WAVE) HIPCHK(ctx, launch_wave(L, st));
+    else HIPCHK(ctx, launch_tile(L, st));
     return BLPG_OK;
   }
   StreamLaunch L{};
@@ -589,8 +598,10 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
       // FP64 fallback kernel of this bin
       Bin& f = fbins[i];
       f.rcap = b.rcap; f.ccap = b.ccap; f.nt = Tuner::stream_threads(b.rcap - 2, b.ccap - 2);
+      const int wv = Tuner::best_wave(b.rcap - 2, b.ccap - 2, BLPG_FP64);
       const int tv = Tuner::best_tile(b.rcap - 2, b.ccap - 2, BLPG_FP64);
-      if (tv >= 0) { f.kernel = BLPG_KERNEL_TILE; f.variant = tv; }
+      if (wv >= 0) { f.kernel = BLPG_KERNEL_WAVE; f.variant = wv; }
+      else if (tv >= 0) { f.kernel = BLPG_KERNEL_TILE; f.variant = tv; }
       else if (stream_lds_bytes(BLPG_FP64, true, f.nt, f.rcap, f.ccap) <= ctx->tuner.lds_limit() &&
                stream_max_blocks_per_cu(BLPG_FP64, true, f.nt, f.rcap, f.ccap) > 0)
         f.kernel = BLPG_KERNEL_LDS;
@@ -1084,12 +1095,16 @@ int blpg_solve_batch(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_resu
 int blpg_tune_set(blpg_ctx* ctx, int32_t precision, int32_t m, int32_t n, int32_t kernel,
                   int32_t param) {
   if (!ctx || precision < 0 || precision > 1 || m < 1 || n < 1 || kernel < BLPG_KERNEL_STREAM ||
-      kernel > BLPG_KERNEL_TILE)
+      kernel > BLPG_KERNEL_WAVE)
     return fail(ctx, BLPG_E_INVALID, "invalid arguments to blpg_tune_set");
   if (kernel == BLPG_KERNEL_TILE &&
       (param < 0 || param >= tile_variant_count(precision) || !Tuner::fits_tile(param, m, n, precision)))
     return fail(ctx, BLPG_E_INVALID, "tile variant %d cannot hold %dx%d", param, m, n);
-  if (kernel != BLPG_KERNEL_TILE && param != 64 && param != 128 && param != 256 && param != 512)
+  if (kernel == BLPG_KERNEL_WAVE &&
+      (param < 0 || param >= wave_variant_count() || !Tuner::fits_wave(param, m, n, precision)))
+    return fail(ctx, BLPG_E_INVALID, "wave variant %d cannot hold %dx%d", param, m, n);
+  if (kernel != BLPG_KERNEL_TILE && kernel != BLPG_KERNEL_WAVE && param != 64 && param != 128 &&
+      param != 256 && param != 512)
     return fail(ctx, BLPG_E_INVALID, "threads per workgroup must be 64, 128, 256 or 512");
   ctx->tuner.set_preference(precision, m, n, kernel, param);
   return BLPG_OK;
@@ -1132,6 +1147,9 @@ int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps
     const int m = g.first.first, n = g.first.second;
     // candidates: every instance that can hold the shape and be resident
     std::vector<std::pair<int, int>> cand;
+    for (int v = 0; v < wave_variant_count(); ++v)
+      if (Tuner::fits_wave(v, m, n, prec) && wave_max_blocks_per_cu(prec, v) > 0)
+        cand.push_back({BLPG_KERNEL_WAVE, v});
     for (int v = 0; v < tile_variant_count(prec); ++v)
       if (Tuner::fits_tile(v, m, n, prec) && tile_max_blocks_per_cu(prec, v) > 0)
         cand.push_back({BLPG_KERNEL_TILE, v});
@@ -1142,13 +1160,14 @@ int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps
       if (stream_max_blocks_per_cu(prec, false, nt, m + 2, n + 2) > 0)
         cand.push_back({BLPG_KERNEL_STREAM, nt});
     }
-    // the static routing's tile variant is timed first and kept unless another instance is
-    // more than 3% faster: closer timings are run-to-run noise, and a choice that flips
-    // between runs moves shapes between bins (and the batch's critical path with them)
-    const int tv0 = Tuner::best_tile(m, n, prec);
-    std::stable_partition(cand.begin(), cand.end(), [&](const std::pair<int, int>& c) {
-      return c.first == BLPG_KERNEL_TILE && c.second == tv0;
-    });
+    // the static route is timed first and kept unless another instance is more than 3%
+    // faster: closer timings are run-to-run noise, and a choice that flips between runs
+    // moves shapes between bins (and the batch's critical path with them)
+    const Route r0 = ctx->tuner.route(m, n, prec, BLPG_KERNEL_AUTO);
+    const std::pair<int, int> c0{r0.kernel, (r0.kernel == BLPG_KERNEL_TILE || r0.kernel == BLPG_KERNEL_WAVE)
+                                                ? r0.variant : r0.nt};
+    std::stable_partition(cand.begin(), cand.end(),
+                          [&](const std::pair<int, int>& c) { return c == c0; });
     double best_ms = 1e30;
     std::pair<int, int> best{-1, -1};
     // the shape's route before the measurement, restored if a candidate fails
@@ -1170,7 +1189,7 @@ int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps
       }
       blpg_batch_free(ctx, B);
       if (rc) break;
-      const bool incumbent_static = best.first == BLPG_KERNEL_TILE && best.second == tv0;
+      const bool incumbent_static = best == c0;
       if (t < best_ms * (incumbent_static ? 0.97 : 1.0)) { best_ms = t; best = c; }
     }
     if (rc) {

@@ -56,6 +56,20 @@ struct TileVariant {
   int rcap() const { return tr * rpt; }
   int ccap() const { return (nw * 64 / tr) * cpt; }
 };
+// Column-per-lane register simplex kernel (kernel_wave.hip): one LP per NW wavefronts,
+// each thread holds whole columns; launched with the TileLaunch arguments (variant = wave
+// variant index).
+struct WaveVariant {
+  int nw, cq, rows;  // waves per LP, column slots per thread, row capacity
+  bool enabled;      // instantiated in this precision
+  int rcap() const { return rows; }
+  int ccap() const { return 1 + 64 * nw * cq; }
+};
+int wave_variant_count();
+WaveVariant wave_variant(int precision, int v);
+int wave_max_blocks_per_cu(int precision, int v);
+hipError_t launch_wave(const TileLaunch& L, hipStream_t st);
+
 int tile_variant_count(int precision);
 int tile_default_variants();  // variants 0..k-1 form the static routing order
 TileVariant tile_variant(int precision, int v);

@@ -1,7 +1,9 @@
 // tuner.hpp -- per-size-bin kernel routing.
 //
 // For each LP shape the tuner picks one of:
-//   TILE   the register-resident kernel (smallest tile variant whose capacity covers
+//   WAVE   the column-per-lane register kernel (smallest variant whose capacity covers
+//          (m+2) x (n+2) in the precision),
+//   TILE   the 2-D register-tile kernel (smallest tile variant whose capacity covers
 //          (m+2) x (n+2) and whose registers fit),
 //   LDS    the streamed kernel with the tableau in LDS (when it fits the CU's LDS),
 //   STREAM the streamed kernel with the tableau in an HBM slab (any size).
@@ -87,7 +89,7 @@ class Tuner {
         if (!*p) break;
         return false;
       }
-      if (pr < 0 || pr > 1 || m < 1 || n < 1 || m > 4096 || n > 4096 || kern < 1 || kern > 3)
+      if (pr < 0 || pr > 1 || m < 1 || n < 1 || m > 4096 || n > 4096 || kern < 1 || kern > 4)
         return false;
       set_preference(pr, m, n, kern, param, ms);
       p += used;
@@ -102,6 +104,7 @@ class Tuner {
     r.nt = stream_threads(m, n);
     const int esz = precision == BLPG_FP64 ? 8 : 4;
     const int tv = best_tile(m, n, precision);
+    const int wv = best_wave(m, n, precision);
     const size_t lds_need =
         stream_lds_bytes(precision, true, r.nt, m + 2, n + 2);
     const bool lds_ok = lds_need <= lds_ && stream_max_blocks_per_cu(precision, true, r.nt, m + 2, n + 2) > 0;
@@ -109,6 +112,10 @@ class Tuner {
     auto it = pref_.find(key(precision, m, n));
     if (forced == BLPG_KERNEL_AUTO && it != pref_.end()) {
       const int k = it->second.first;
+      if (k == BLPG_KERNEL_WAVE && it->second.second >= 0 &&
+          it->second.second < wave_variant_count() && fits_wave(it->second.second, m, n, precision)) {
+        r.kernel = k; r.variant = it->second.second; return r;
+      }
       if (k == BLPG_KERNEL_TILE && it->second.second >= 0 &&
           it->second.second < tile_variant_count(precision) &&
           fits_tile(it->second.second, m, n, precision)) {
@@ -117,18 +124,46 @@ class Tuner {
       if (k == BLPG_KERNEL_LDS && lds_ok) { r.kernel = k; if (it->second.second) r.nt = it->second.second; return r; }
       if (k == BLPG_KERNEL_STREAM) { r.kernel = k; if (it->second.second) r.nt = it->second.second; return r; }
     }
-    if ((forced == BLPG_KERNEL_AUTO || forced == BLPG_KERNEL_TILE) && tv >= 0) {
+    if ((forced == BLPG_KERNEL_AUTO || forced == BLPG_KERNEL_WAVE) && wv >= 0) {
+      r.kernel = BLPG_KERNEL_WAVE;
+      r.variant = wv;
+      return r;
+    }
+    if ((forced == BLPG_KERNEL_AUTO || forced == BLPG_KERNEL_TILE || forced == BLPG_KERNEL_WAVE) &&
+        tv >= 0) {
       r.kernel = BLPG_KERNEL_TILE;
       r.variant = tv;
       return r;
     }
-    if ((forced == BLPG_KERNEL_AUTO || forced == BLPG_KERNEL_LDS || forced == BLPG_KERNEL_TILE) &&
+    if ((forced == BLPG_KERNEL_AUTO || forced == BLPG_KERNEL_LDS || forced == BLPG_KERNEL_TILE ||
+         forced == BLPG_KERNEL_WAVE) &&
         lds_ok) {
       r.kernel = BLPG_KERNEL_LDS;
       return r;
     }
     r.kernel = BLPG_KERNEL_STREAM;
     return r;
+  }
+
+  static bool fits_wave(int v, int m, int n, int precision) {
+    const WaveVariant w = wave_variant(precision, v);
+    return w.enabled && w.rcap() >= m + 2 && w.ccap() >= n + 2;
+  }
+
+  // smallest-capacity wave variant that holds the LP (ties: fewer wavefronts per LP)
+  static int best_wave(int m, int n, int precision) {
+    int best = -1;
+    long best_cap = 0;
+    int best_nw = 0;
+    for (int v = 0; v < wave_variant_count(); ++v) {
+      if (!fits_wave(v, m, n, precision)) continue;
+      const WaveVariant w = wave_variant(precision, v);
+      const long cap = (long)w.rcap() * w.ccap();
+      if (best < 0 || cap < best_cap || (cap == best_cap && w.nw < best_nw)) {
+        best = v; best_cap = cap; best_nw = w.nw;
+      }
+    }
+    return best;
   }
 
   static bool fits_tile(int v, int m, int n, int precision) {

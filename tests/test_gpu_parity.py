@@ -32,7 +32,7 @@ def sample(cfg, count):
     return lps
 
 
-@pytest.mark.parametrize("kernel", ["tile", "lds", "stream"])
+@pytest.mark.parametrize("kernel", ["wave", "tile", "lds", "stream"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("cfg", [1, 2, 3])
 def test_bit_exact_vs_oracle(cfg, precision, kernel):
