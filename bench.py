@@ -1,26 +1,36 @@
 #!/usr/bin/env python3
 """bench.py -- batched LP throughput on one MI355X.
 
-Workload (one "step"): the mixed-size batch of 4096 inequality LPs, 1024 each at
-50x48, 56x97, 74x83 and 105x103 (A ~ U[-1,1) with 40% zeroed, b ~ U[0,10) with 10% of
-rows exactly 0, c ~ N(0,1), 5% planted unbounded; seed 2), already resident in HBM,
-solved through the C ABI (blpg_batch_solve: size-bin kernels, FP64 re-solve + KKT check,
-FP64 fallback) with the results copied back to the host (blpg_batch_download).
+Headline workload (one "step"): the mixed-size batch of 4096 inequality LPs, 1024 each at
+50x48, 56x97, 74x83 and 105x103 (A ~ U[-1,1) with 40% zeroed, b ~ U[0,10) with 10% of rows
+exactly 0, c ~ N(0,1), 5% planted unbounded; seed 2), solved through the C ABI
+(blpg_batch_solve: size-bin kernels, FP64 re-solve + KKT check, FP64 fallback, one captured
+hipGraph) with the results copied back to the host (blpg_batch_fetch).
 
 `value` counts the LPs of the FP32-tableau run whose status equals the all-FP64 run's and,
-when optimal, whose objective is within 1e-6 * max(1, |obj_fp64|); the all-FP64 run is
-timed the same way and reported beside it.  The roofline object covers the dominant
-solve kernel: algorithmic bytes per launch (sum over its LPs of pivots * 2*(m+1)*(n+1)*s,
-one more row per phase-I pivot, plus the LP's input and output bytes) over its average
-HIP-event duration.  The CPU baseline is the oracle's FP64 replay of the same rule over
-the host's cores on the same LPs.
+when optimal, whose objective is within 1e-6 * max(1, |obj_fp64|), per second, with the
+inputs resident in HBM when the timed region starts.  `end_to_end` times the same steps
+with the inputs' host-to-device copy inside them (the reference's timed ComputeLP: HtoD +
+kernels + DtoH, simplex.cu:944-953): each step repacks the batch into its pinned staging
+(host work, the reference's untimed setConstratint) and then times solve (whose graph copies
+every size bin's inputs just before its kernel) + fetch.
+
+The roofline object covers the dominant solve kernel (most algorithmic FLOPs): FLOPs per
+launch (sum over its LPs of pivots * 2*(m+1)*(n+1), one more row per phase-I pivot) over its
+HIP-event duration on its own stream, against the FP32 vector peak: the tableau never leaves
+registers, so the kernel is bound by the issue latency of each pivot's dependent chain
+(reductions, LDS round trips, the FMA sweep), not by HBM or MFMA.  Its HBM fraction uses
+the per-launch traffic measured with rocprofv3 counters (profiles/traffic.json, by kernel
+route); the cycles per pivot of the bin's slowest LP bound the bin's duration.
+
+The CPU baseline is the oracle's FP64 replay of the same rule over all the host's cores
+(the affinity mask), on the same LPs, packed once outside the timing.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
 import sys
@@ -36,11 +46,21 @@ SHAPES = [(50, 48), (56, 97), (74, 83), (105, 103)]
 PER_SHAPE = 1024
 SEED = 2
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
-# same counters on a plain 256 MiB device copy (FETCH x2.000, WRITE x1.000); the commands and
-# raw counts are in profiles/tile_fp32_105x103_mixed_batch.md.
-PROFILED_TRAFFIC = {("tile", 4, "fp32", 105, 103, 1024): 152.3e6}
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+FP32_FMA_TFLOPS = 78.6     # one non-packed FMA per lane per clock (the kernels' only form)
+CLOCK_GHZ = 2.4            # MI355X peak engine clock
+KNAMES = {1: "stream", 2: "lds", 3: "tile", 4: "wave"}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def build_batch(sa):
@@ -57,11 +77,32 @@ def autotune(sa, precision, lps, reps=2):
     table = S.autotune(lps, reps=reps)
     text = S.tune_export()
     S.close()
-    names = {1: "stream", 2: "lds", 3: "tile"}
-    return text, [[m, n, names[k], q, round(ms, 4)] for (_, m, n, k, q, ms) in table]
+    return text, [[m, n, KNAMES[k], q, round(ms, 4)] for (_, m, n, k, q, ms) in table]
 
 
-def timed_run(sa, precision, lps, steps, warmup, verify=True, model=None, tune=None):
+def stage_timings(S, batch, reps):
+    """Per-stage timings from eager solves (HIP events on each bin's stream)."""
+    S.set(use_graph=0)
+    infos = []
+    for _ in range(reps):
+        batch.solve()
+        batch.sync()
+        infos.append(batch.info())
+    S.set(use_graph=1)
+    nb = len(infos[0]["bins"])
+    return {"bins": infos[0]["bins"],
+            "bins_ms": [float(np.median([inf["bins"][i]["kernel_ms"] for inf in infos]))
+                        for i in range(nb)],
+            "verify_ms": float(np.median([i["verify_ms"] for i in infos])),
+            "fallback_ms": float(np.median([i["fallback_ms"] for i in infos])),
+            "warmstart_ms": float(np.median([i["warmstart_ms"] for i in infos])),
+            "gpu_total_ms_eager": float(np.median([i["total_ms"] for i in infos]))}
+
+
+def timed_run(sa, precision, lps, steps, warmup, verify=True, model=None, tune=None,
+              e2e=False):
+    """K timed steps of solve + fetch (inputs resident).  With e2e, K more steps that each
+    repack the batch (untimed) and then time solve (with its input copies) + fetch."""
     S = model.solver if model is not None else sa.Solver(precision=precision, verify=verify)
     if model is not None:
         S.set(precision=precision, verify=int(verify))
@@ -74,36 +115,84 @@ def timed_run(sa, precision, lps, steps, warmup, verify=True, model=None, tune=N
         batch.solve()  # the first solve captures the batch's hipGraph; later ones replay it
         batch.fetch_raw()
     batch.sync()
+    per_step = []
     t0 = time.perf_counter()
-    totals = []
     for _ in range(steps):
+        ts = time.perf_counter()
         batch.solve()
         batch.fetch_raw()  # waits for the solve; results land in pinned host memory
-        totals.append(batch.info()["total_ms"])
+        per_step.append(time.perf_counter() - ts)
     batch.sync()
     t1 = time.perf_counter()
     out = batch.fetch()
-    # per-stage timings (untimed for the throughput): eager solves with timing events
-    S.set(use_graph=0)
-    infos = []
-    for _ in range(max(2, min(steps, 5))):
+    meta = stage_timings(S, batch, max(3, min(steps, 5)))
+    meta["median_step_s"] = float(np.median(per_step))
+    meta["fallback_count"] = int(out.fallback.sum())
+    if e2e:
+        batch.repack(lps)  # warm the copy-in graph
         batch.solve()
-        batch.sync()
-        infos.append(batch.info())
-    S.set(use_graph=1)
-    nb = len(infos[0]["bins"])
-    bins_ms = [float(np.mean([inf["bins"][i]["kernel_ms"] for inf in infos])) for i in range(nb)]
-    meta = {"bins": infos[0]["bins"], "bins_ms": bins_ms,
-            "verify_ms": float(np.mean([i["verify_ms"] for i in infos])),
-            "fallback_ms": float(np.mean([i["fallback_ms"] for i in infos])),
-            "gpu_total_ms": float(np.mean(totals)),
-            "gpu_total_ms_eager": float(np.mean([i["total_ms"] for i in infos])),
-            "warmstart_ms": float(np.mean([i["warmstart_ms"] for i in infos])),
-            "fallback_count": int(out.fallback.sum())}
+        batch.fetch_raw()
+        pack_s, step_s = [], []
+        for _ in range(steps):
+            tp = time.perf_counter()
+            batch.repack(lps)
+            ts = time.perf_counter()
+            batch.solve()
+            batch.fetch_raw()
+            te = time.perf_counter()
+            pack_s.append(ts - tp)
+            step_s.append(te - ts)
+        r2 = batch.fetch()
+        same = bool(np.array_equal(r2.status, out.status) and np.array_equal(r2.objective, out.objective))
+        meta["e2e"] = {"ms_per_step": float(np.mean(step_s)) * 1e3,
+                       "median_ms": float(np.median(step_s)) * 1e3,
+                       "pack_ms": float(np.median(pack_s)) * 1e3, "same_results": same}
     batch.free()
     if model is None:
         S.close()
     return out, (t1 - t0) / steps, meta
+
+
+def accurate(sa, r32, r64):
+    same = r32.status == r64.status
+    opt = r64.status == sa.OPTIMAL
+    rel = np.zeros(len(r64.status))
+    rel[opt] = np.abs(r32.objective[opt] - r64.objective[opt]) / np.maximum(1.0, np.abs(r64.objective[opt]))
+    decided = np.isin(r64.status, [sa.OPTIMAL, sa.INFEASIBLE, sa.UNBOUNDED])
+    good = same & (~opt | (rel <= 1e-6)) & decided
+    return int(good.sum()), float(rel.max()) if len(rel) else 0.0
+
+
+def algorithmic_flops(lps, res, idx):
+    """Sum over LPs of pivots * 2*(m+1)*(n+1) (+1 row per phase-I pivot)."""
+    tot = 0
+    for k in idx:
+        m, n = lps[k].m, lps[k].n
+        p = int(res.pivots[k])
+        p1 = int(res.phase1_pivots[k])
+        tot += (p - p1) * 2 * (m + 1) * (n + 1) + p1 * 2 * (m + 2) * (n + 1)
+    return tot
+
+
+def traffic_table():
+    try:
+        with open(os.path.join(HERE, "profiles", "traffic.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def cpu_baseline(lps, threads, reps):
+    """The oracle over the whole batch, packed once; `reps` timed runs, median."""
+    import oracle
+    pk = oracle.pack_batch(lps)
+    oracle.solve_packed(pk, threads)  # warm-up
+    dts, r = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = oracle.solve_packed(pk, threads)
+        dts.append(time.perf_counter() - t0)
+    return r, float(np.median(dts)), dts
 
 
 def warmstart_leg(sa, steps, warmup, do_tune=True):
@@ -132,20 +221,17 @@ def warmstart_leg(sa, steps, warmup, do_tune=True):
     cold, t_cold, mc = timed_run(sa, "fp64", test, steps, warmup, verify=False, tune=t64)
     coldv, t_coldv, mcv = timed_run(sa, "fp64", test, steps, warmup, verify=True, tune=t64)
     warm, t_warm, mw = timed_run(sa, "fp32", test, steps, warmup, verify=True, model=ws, tune=t32)
-    opt = cold.status == sa.OPTIMAL
-    rel = np.abs(warm.objective[opt] - cold.objective[opt]) / np.maximum(1, np.abs(cold.objective[opt]))
-    good = (warm.status == cold.status) & np.where(opt, np.abs(warm.objective - cold.objective)
-                                                   / np.maximum(1, np.abs(cold.objective)) <= 1e-6, True)
+    good, rel = accurate(sa, warm, cold)
     ws.close()
     S.close()
     return {
         "workload": "64 parametric families x 512 unseen instances (105x103), seed 4",
-        "warm_fp32_lps_per_s": int(good.sum()) / t_warm, "warm_ms_per_step": t_warm * 1e3,
+        "warm_fp32_lps_per_s": good / t_warm, "warm_ms_per_step": t_warm * 1e3,
         "cold_fp64_lps_per_s": len(test) / t_cold, "cold_ms_per_step": t_cold * 1e3,
         "cold_fp64_checked_lps_per_s": len(test) / t_coldv,
-        "speedup_vs_cold_fp64": (int(good.sum()) / t_warm) / (len(test) / t_cold),
-        "speedup_vs_cold_fp64_checked": (int(good.sum()) / t_warm) / (len(test) / t_coldv),
-        "accurate": int(good.sum()), "max_rel_obj_err": float(rel.max()) if opt.any() else 0.0,
+        "speedup_vs_cold_fp64": (good / t_warm) / (len(test) / t_cold),
+        "speedup_vs_cold_fp64_checked": (good / t_warm) / (len(test) / t_coldv),
+        "accurate": good, "max_rel_obj_err": rel,
         "pivots_warm": float(warm.pivots.mean()), "pivots_cold": float(cold.pivots.mean()),
         "fallbacks": mw["fallback_count"], "inference_ms": mw["warmstart_ms"],
         "warm_kernel_ms": mw["bins_ms"], "warm_verify_ms": mw["verify_ms"],
@@ -154,48 +240,50 @@ def warmstart_leg(sa, steps, warmup, do_tune=True):
     }
 
 
-def algorithmic_bytes(lps, res, idx, s):
-    """Sum over LPs of pivots * 2*(m+1)*(n+1)*s (+1 row in phase I) + I/O bytes."""
-    tot = 0
-    for k in idx:
-        m, n = lps[k].m, lps[k].n
-        p = int(res.pivots[k]); p1 = int(res.phase1_pivots[k])
-        tot += (p - p1) * 2 * (m + 1) * (n + 1) * s + p1 * 2 * (m + 2) * (n + 1) * s
-        tot += (m * n + m + n) * 8 + (n + m) * (8 + 4) + 16
-    return tot
+def config1_leg(sa, steps, warmup, threads):
+    """The reference's own CPU-runnable case: 1024 LPs 27x32 (A ~ U[0,1) 30% zeroed,
+    b ~ U[1,10), c ~ U[0,1), seed 0), all-FP64, against the CPU oracle."""
+    lps = sa.gen_config(1, 0, 27, 32, 1024, batch_count=1024)
+    r, t, meta = timed_run(sa, "fp64", lps, steps, warmup, verify=False)
+    rc, dtc, _ = cpu_baseline(lps, threads, 20)
+    return {"workload": "1024 LPs 27x32, seed 0, FP64 tableau", "lps_per_s": len(lps) / t,
+            "ms_per_step": t * 1e3, "bins_ms": meta["bins_ms"],
+            "cpu_lps_per_s": len(lps) / dtc, "cpu_threads": threads,
+            "status_agreement_cpu": float(np.mean(rc["status"] == r.status)),
+            "objective_bit_equal_cpu": bool(np.array_equal(rc["objective"], r.objective)),
+            "pivots_mean": float(r.pivots.mean())}
 
 
-def algorithmic_flops(lps, res, idx):
-    tot = 0
-    for k in idx:
-        m, n = lps[k].m, lps[k].n
-        p = int(res.pivots[k]); p1 = int(res.phase1_pivots[k])
-        tot += (p - p1) * 2 * (m + 1) * (n + 1) + p1 * 2 * (m + 2) * (n + 1)
-    return tot
-
-
-def cpu_baseline(lps, threads, min_wall=0.75):
-    """The oracle over the whole batch, repeated until about min_wall seconds of wall time
-    (10-20 s of CPU time on 16 threads).  Returns the last result, the wall time per batch
-    and the number of repetitions."""
-    import oracle
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        r = oracle.solve_batch(lps, threads=threads)
-        reps += 1
-        if time.perf_counter() - t0 >= min_wall:
-            break
-    dt = (time.perf_counter() - t0) / reps
-    return r, dt, reps
+def config3_leg(sa, steps, warmup, do_tune=True):
+    """Two-phase batch: 8192 LPs each at 174x142 and 205x203 (30% '=' rows, b ~ N(0,5),
+    3% planted infeasible, seed 3), FP32 tableau with FP64 phase-I decisions and check,
+    against the all-FP64 run; routes measured on 1024 LPs of each shape."""
+    lps = []
+    for si, (m, n) in enumerate([(174, 142), (205, 203)]):
+        lps += sa.gen_config(3, 3, m, n, 8192, shape_idx=si, batch_count=8192)
+    t32 = t64 = None
+    tab = {}
+    if do_tune:
+        sample = lps[:1024] + lps[8192:9216]
+        t64, tab["fp64"] = autotune(sa, "fp64", sample, reps=1)
+        t32, tab["fp32"] = autotune(sa, "fp32", sample, reps=1)
+    r32, t32, m32 = timed_run(sa, "fp32", lps, steps, warmup, tune=t32)
+    r64, t64, m64 = timed_run(sa, "fp64", lps, steps, warmup, tune=t64)
+    good, rel = accurate(sa, r32, r64)
+    mix = {sa.STATUS_NAMES.get(int(s), str(s)): int(c)
+           for s, c in zip(*np.unique(r64.status, return_counts=True))}
+    return {"workload": "16384 two-phase LPs, 8192 each at 174x142 and 205x203, seed 3",
+            "fp32_lps_per_s": good / t32, "fp32_ms_per_step": t32 * 1e3,
+            "fp64_lps_per_s": len(lps) / t64, "fp64_ms_per_step": t64 * 1e3,
+            "accurate": good, "status_mix": mix, "fp32_bins_ms": m32["bins_ms"],
+            "fp32_fallbacks": m32["fallback_count"], "pivots_mean": float(r64.pivots.mean()),
+            "fp64_bins_ms": m64["bins_ms"], "tuning": tab}
 
 
 def serving_leg(sa, tune32=None, seconds=2.0, rate=50000.0):
     """Serving loop: Poisson arrivals at `rate` LPs/s of single LPs, shapes uniform over
     27x32, 50x48, 74x83, 105x103, 205x203 (generators as the first two workloads, seed 5),
-    FP32 tableau + FP64 check; latency counted from each scheduled arrival.  Two seconds of
-    arrivals: the drain after the last arrival (tens of ms, the largest shape's latency) is
-    a fixed cost that a short window would count as lost throughput."""
+    FP32 tableau + FP64 check; latency counted from each scheduled arrival."""
     shapes = [(27, 32), (50, 48), (74, 83), (105, 103), (205, 203)]
     pools = [sa.gen_config(5, 5, m, n, 512, shape_idx=i, batch_count=1024)
              for i, (m, n) in enumerate(shapes)]
@@ -225,8 +313,6 @@ def serving_leg(sa, tune32=None, seconds=2.0, rate=50000.0):
     return {"workload": f"Poisson {rate:.0f} LPs/s for {seconds} s, shapes uniform over "
                         "27x32, 50x48, 74x83, 105x103, 205x203, seed 5",
             "offered": count, "completed": int((res["status"] != sa.UNDECIDED).sum()),
-            # sustained rate: LPs over the span from the replay's start to the last completion
-            # (arrival + latency; the Python marshalling of the requests is outside it)
             "lps_per_s": count / (float(np.max(arrival + lat)) * 1e-6),
             "span_s": float(np.max(arrival + lat)) * 1e-6, "wall_s": wall,
             "p50_us": float(np.percentile(lat, 50)), "p99_us": float(np.percentile(lat, 99)),
@@ -246,6 +332,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-warmstart", action="store_true",
                     help="skip the learned warm start leg (parametric families)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the config-1 and two-phase (config-3) legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: LPs are independent, so the batch shards "
@@ -257,56 +345,54 @@ def main():
 
     lps = build_batch(sa)
     count = len(lps)
+    threads = len(os.sched_getaffinity(0))
     tune64 = tune32 = None
     tuning = {}
     if not args.no_tune:
         tune64, tuning["fp64"] = autotune(sa, "fp64", lps)
         tune32, tuning["fp32"] = autotune(sa, "fp32", lps)
     r64, t64, meta64 = timed_run(sa, "fp64", lps, args.steps, args.warmup, tune=tune64)
-    r32, t32, meta32 = timed_run(sa, "fp32", lps, args.steps, args.warmup, tune=tune32)
-
-    # accuracy of the FP32 path against the all-FP64 run
-    same = r32.status == r64.status
-    opt = r64.status == sa.OPTIMAL
-    rel = np.zeros(count)
-    rel[opt] = np.abs(r32.objective[opt] - r64.objective[opt]) / np.maximum(1.0, np.abs(r64.objective[opt]))
-    good = same & (~opt | (rel <= 1e-6))
-    decided64 = np.isin(r64.status, [sa.OPTIMAL, sa.INFEASIBLE, sa.UNBOUNDED])
-    n_good = int(np.sum(good & decided64))
+    r32, t32, meta32 = timed_run(sa, "fp32", lps, args.steps, args.warmup, tune=tune32, e2e=True)
+    n_good, max_rel = accurate(sa, r32, r64)
     value = n_good / t32
 
-    # roofline of the dominant solve kernel of the FP32 run: the bin with the most
-    # algorithmic bytes (its duration comes from the HIP events of the eager solves, where
-    # the bins share the GPU as they do in the timed graph replays)
+    # roofline of the dominant solve kernel of the FP32 run (most algorithmic FLOPs); its
+    # duration is the median HIP-event time of its launch in the eager solves, where the
+    # bins share the GPU as in the timed graph replays
     def bin_lps(b):
         shape = (meta32["bins"][b]["rcap"] - 2, meta32["bins"][b]["ccap"] - 2)
         return shape, [k for k, lp in enumerate(lps) if (lp.m, lp.n) == shape]
-    bi = int(np.argmax([algorithmic_bytes(lps, r32, bin_lps(b)[1], 4)
-                        for b in range(len(meta32["bins"]))]))
+    flops = [algorithmic_flops(lps, r32, bin_lps(b)[1]) for b in range(len(meta32["bins"]))]
+    bi = int(np.argmax(flops))
     bin_shape, idx = bin_lps(bi)
-    bytes_launch = algorithmic_bytes(lps, r32, idx, 4)
-    flops_launch = algorithmic_flops(lps, r32, idx)
     ms = meta32["bins_ms"][bi]
-    achieved = bytes_launch / (ms * 1e-3) / 1e9
-    kname = {1: "stream", 2: "lds", 3: "tile"}[meta32["bins"][bi]["kernel"]]
+    achieved = flops[bi] / (ms * 1e-3) / 1e12
+    kname = KNAMES[meta32["bins"][bi]["kernel"]]
     kparam = meta32["bins"][bi]["param"]
+    route = f"{kname}[{kparam}] fp32 {bin_shape[0]}x{bin_shape[1]} x{len(idx)}"
+    traffic = traffic_table().get(route)
+    maxpiv = int(max(r32.pivots[k] for k in idx))
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        rc, dtc, reps = cpu_baseline(lps, threads)
+        rc, dtc, dts = cpu_baseline(lps, threads, 15)
         agree = float(np.mean(rc["status"] == r64.status))
         cpu = {"value": count / dtc, "unit": "LPs/s", "cores": threads, "kind": "port",
-               "sample": f"the full {count}-LP batch {reps} times, oracle FP64 replay of the "
-                         f"same pivot rule (C, -O3 AVX2/FMA, OpenMP), {dtc * reps:.2f} s wall "
-                         f"({dtc * reps * threads:.0f} thread-seconds); status agreement with "
-                         f"GPU FP64 {agree:.4f}"}
+               "sample": f"the full {count}-LP batch, 15 timed runs (median {dtc * 1e3:.1f} ms, "
+                         f"{sum(dts) * threads:.0f} thread-seconds), oracle FP64 replay of the "
+                         f"same pivot rule (C, -O3, OpenMP over {threads} threads = the affinity "
+                         f"mask) on {cpu_model()}; status agreement with GPU FP64 {agree:.4f}; "
+                         "own-CPU baseline, not GLPK"}
 
+    extra1 = extra3 = None
+    if not args.no_extra:
+        extra1 = config1_leg(sa, args.steps, args.warmup, threads)
+        extra3 = config3_leg(sa, max(2, args.steps // 3), 1, do_tune=not args.no_tune)
     wsleg = (None if args.no_warmstart else
              warmstart_leg(sa, args.steps, args.warmup, do_tune=not args.no_tune))
-
     serve = None if args.no_serve else serving_leg(sa, tune32)
 
+    e2e = meta32["e2e"]
     line = {
         "metric": METRIC, "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t32 * 1e3, "higher_is_better": True,
@@ -316,24 +402,40 @@ def main():
                                "KKT check + FP64 fallback; inputs resident in HBM, results "
                                "to host",
                    "lps": count, "seed": SEED},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": PROFILED_TRAFFIC.get((kname, kparam, "fp32") + bin_shape + (len(idx),)),
-                     "kernel": f"{kname}[{kparam}] fp32 bin {bin_shape[0]}x{bin_shape[1]}",
-                     "launch_ms": ms, "algorithmic_bytes": bytes_launch,
-                     "fp64_valu_equiv_tflops": flops_launch / (ms * 1e-3) / 1e12},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic, "kernel": route, "launch_ms": ms,
+                     "algorithmic_flops": flops[bi],
+                     "frac_of_fma_rate": achieved / FP32_FMA_TFLOPS,
+                     "hbm_frac": (traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+                     "critical_lp_pivots": maxpiv,
+                     "cycles_per_pivot_critical_lp": ms * 1e-3 * CLOCK_GHZ * 1e9 / maxpiv,
+                     "note": "latency-bound: the tableau stays in registers; frac is FLOPs "
+                             "over the FP32 vector peak (157.3 TFLOP/s); frac_of_fma_rate "
+                             "over the non-packed FMA rate (78.6 TFLOP/s)"},
         "cpu_baseline": cpu,
+        "end_to_end": {"lps_per_s": n_good / (e2e["ms_per_step"] * 1e-3),
+                       "ms_per_step": e2e["ms_per_step"], "median_ms": e2e["median_ms"],
+                       "pack_ms_untimed": e2e["pack_ms"], "same_results": e2e["same_results"],
+                       "vs_cpu": ((n_good / (e2e["ms_per_step"] * 1e-3)) / cpu["value"]) if cpu else None,
+                       "note": "HtoD of every bin's inputs inside each step (graph memcpy nodes "
+                               "from pinned staging, overlapped with earlier bins' kernels) + "
+                               "solve + DtoH; the repack into staging is untimed host work"},
         "fp32": {"lps_per_s_all": count / t32, "accurate": n_good, "ms_per_step": t32 * 1e3,
+                 "median_step_ms": meta32["median_step_s"] * 1e3,
                  "fallbacks": meta32["fallback_count"], "bins_ms": meta32["bins_ms"],
                  "verify_ms": meta32["verify_ms"], "fallback_ms": meta32["fallback_ms"],
-                 "gpu_ms": meta32["gpu_total_ms"], "max_rel_obj_err": float(rel.max())},
+                 "gpu_ms_eager": meta32["gpu_total_ms_eager"], "max_rel_obj_err": max_rel,
+                 "vs_cpu": (value / cpu["value"]) if cpu else None},
         "fp64": {"lps_per_s": count / t64, "ms_per_step": t64 * 1e3,
                  "bins_ms": meta64["bins_ms"], "verify_ms": meta64["verify_ms"],
-                 "gpu_ms": meta64["gpu_total_ms"]},
+                 "gpu_ms_eager": meta64["gpu_total_ms_eager"]},
         "status_mix": {sa.STATUS_NAMES.get(int(s), str(s)): int(c)
                        for s, c in zip(*np.unique(r64.status, return_counts=True))},
         "pivots_mean": float(np.mean(r64.pivots)),
         "tuning": tuning,
+        "config1": extra1,
+        "two_phase": extra3,
         "warmstart": wsleg,
         "serving": serve,
     }

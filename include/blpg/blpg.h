@@ -151,8 +151,9 @@ void blpg_batch_free(blpg_ctx *ctx, blpg_batch *batch);
  * uploaded batch: host work only, into the batch's pinned staging, like the reference's
  * setConstratint building its pinned tableau (simplex.cu:463-550).  The next
  * blpg_batch_solve starts with the host-to-device copy of the inputs, as ComputeLP does
- * (simplex.cu:944-953): each size bin's inputs on its own stream just before its kernel,
- * so later bins' copies overlap earlier bins' solves.  Two LPs may share a new array only
+ * (simplex.cu:944-953): each size bin's inputs on its own stream just before its kernel
+ * (largest bin first, one copy after the other), so later bins' copies overlap earlier
+ * bins' solves; that solve is enqueued eagerly, not from the captured graph.  Two LPs may share a new array only
  * where the uploaded batch shared one (the upload stores each distinct array once). */
 int blpg_batch_repack(blpg_ctx *ctx, blpg_batch *batch, const blpg_lp *lps, int64_t count);
 

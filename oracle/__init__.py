@@ -158,6 +158,11 @@ def kkt_check(A, b, c, basis, sense=None) -> Optional[dict]:
 
 def solve_batch(lps, threads: int = 1, precision: str = "fp64") -> dict:
     """FP64 replay of a whole batch over `threads` OpenMP threads (CPU baseline)."""
+    return solve_packed(pack_batch(lps), threads)
+
+
+def pack_batch(lps) -> dict:
+    """The batch in the flat layout solve_packed takes (done once, outside any timing)."""
     k = len(lps)
     pool_parts, spool_parts = [], []
     a_off = np.zeros(k, np.int64); b_off = np.zeros(k, np.int64)
@@ -177,6 +182,15 @@ def solve_batch(lps, threads: int = 1, precision: str = "fp64") -> dict:
             spos += lp.sense.size
     pool = np.concatenate(pool_parts)
     spool = np.concatenate(spool_parts) if spool_parts else np.zeros(1, np.int8)
+    return {"k": k, "m": m, "n": n, "a_off": a_off, "b_off": b_off, "c_off": c_off,
+            "s_off": s_off, "pool": pool, "spool": spool}
+
+
+def solve_packed(pk: dict, threads: int = 1, precision: str = "fp64") -> dict:
+    """FP64 replay of a packed batch over `threads` OpenMP threads."""
+    k, m, n = pk["k"], pk["m"], pk["n"]
+    a_off, b_off, c_off, s_off = pk["a_off"], pk["b_off"], pk["c_off"], pk["s_off"]
+    pool, spool = pk["pool"], pk["spool"]
     mm, nn = int(m.max()), int(n.max())
     d = defaults(precision, mm, nn)
     d["max_iter"] = 0  # per-LP default below

@@ -111,7 +111,7 @@ struct blpg_batch {
   int nev = 0;
   bool timed = false;        // last solve recorded per-stage timing events
   hipEvent_t copied[16] = {};  // per bin: its inputs are on the device (solves with a copy)
-  // captured solves: [0] inputs resident, [1] inputs copied from the staging first
+  // captured solve (slot 0: inputs resident; solves that copy inputs first run eagerly)
   hipGraph_t graph[2] = {};
   hipGraphExec_t gexec[2] = {};
   blpg_options g_opt[2] = {};  // options / model / workspace generation each graph holds
@@ -138,6 +138,11 @@ int set_free_error(int code, const char* fmt, ...);
 
 // results of slots 0..k-1 of a batch (abi.cpp; the serving arenas use it)
 int batch_fetch_prefix(blpg_ctx* ctx, blpg_batch* B, int64_t k, blpg_batch_results* R);
+
+// Batches with warm-start lists crash in blocks in the register-tile kernel: its bins
+// routed to the column-per-lane kernel (which crashes one target at a time) move to the
+// tile variant that holds their shape (abi.cpp).
+void warm_route(blpg_batch* B);
 
 // warm-start inference of a batch's attached model (warmstart.cpp)
 int ws_run_batch(blpg_ctx* ctx, blpg_batch* B, hipStream_t st);

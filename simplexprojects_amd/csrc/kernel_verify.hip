@@ -19,9 +19,11 @@
 //           a basis too ill-conditioned for the refinement stalls with a residual of the
 //           data's own scale (and a huge x or y that would pass the relative tests);
 //   OPTIMAL   objective c_S^T x_S, the verified x written to the LP's x;
-//   UNBOUNDED the entering variable e's ray d = B^-1 a'_e (refined the same way) must
-//             satisfy d <= tol (1 + |d|) (|d| on basic '=' slacks), residual <= tol (1 +
-//             |A|_max |d|), and c_e - y^T a'_e > tol s_D.
+//   UNBOUNDED a feasible point (the origin when b' >= 0 on inequality rows and b = 0 on
+//             '=' rows, else the final basis, which must pass the primal test) and the
+//             entering variable e's ray d = B^-1 a'_e (refined the same way) with
+//             d <= tol (1 + |d|) (|d| on basic '=' slacks), residual <= tol (1 +
+//             |A|_max |d|), and gain c_e - c_B^T d > tol (1 + |c|_inf (1 + |d|_inf)).
 // FP32 results that fail (or FP32 INFEASIBLE whose phase-I objective is not clearly
 // negative, or iteration-limit / numeric statuses) are appended to the fallback list.
 //
@@ -494,100 +496,105 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
     const bool pconv = rres <= sq_tol * (1.0 + bm);
     keep_basis = keep_basis || pconv;
 
-    // ---- y: rho = c_B - B^T y (one thread per basic structural), y += B~^-T rho (one
-    //      thread per nonbasic slack: its tableau column against rho) ----
-    for (int p = tid; p < m; p += NT) y[p] = colp[p] >= 0 ? (double)T[colp[p]] : 0.0;
-    __syncthreads();
-    double ym = 0, rres_d = 0, prev = DBL_MAX;
-    for (int sweep = 0;; ++sweep) {
-      double rl = 0;
-      if (staged) {
-        colsum(AB, y, [&](int a, double sum) {
-          const double val = cS[a] - sum;
-          rho[Srow[a]] = val;
-          rl = fmax(rl, fabs(val));
-        });
-      } else {
-        for (int a = tid; a < k; a += NT) {
-          double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-          int p = 0;
-          const int v = S[a];
-          for (; p + 4 <= m; p += 4) {
-            s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
-            s1 = fma(sg(p + 1) * A[(int64_t)(p + 1) * n + v], y[p + 1], s1);
-            s2 = fma(sg(p + 2) * A[(int64_t)(p + 2) * n + v], y[p + 2], s2);
-            s3 = fma(sg(p + 3) * A[(int64_t)(p + 3) * n + v], y[p + 3], s3);
-          }
-          for (; p < m; ++p) s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
-          const double val = cS[a] - ((s0 + s1) + (s2 + s3));
-          rho[Srow[a]] = val;
-          rl = fmax(rl, fabs(val));
-        }
-      }
-      for (int p = tid; p < m; p += NT) {
-        const int q = rowof[n + p];
-        if (q >= 0) {
-          rho[q] = -y[p];
-          rl = fmax(rl, fabs(y[p]));
-        }
-      }
-      rres_d = rl;
-      ym = 0;
-      for (int p = tid; p < m; p += NT) ym = fmax(ym, fabs(y[p]));
-      bmax2<NT>(rres_d, ym, red);
-      if (sweep == kMaxSweeps || rres_d <= 1e-14 * (1.0 + cm + am * ym) || rres_d > 0.9 * prev) break;
-      prev = rres_d;
-      if (staged) {
-        colsum(TB, rho, [&](int a, double sum) { y[Pn[a]] += sum; });
-      } else {
-        for (int a = tid; a < k; a += NT) {
-          double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-          int i = 0;
-          const int j = Pcol[a];
-          for (; i + 4 <= m; i += 4) {
-            s0 = fma((double)T[(int64_t)(2 + i) * C + j], rho[i], s0);
-            s1 = fma((double)T[(int64_t)(3 + i) * C + j], rho[i + 1], s1);
-            s2 = fma((double)T[(int64_t)(4 + i) * C + j], rho[i + 2], s2);
-            s3 = fma((double)T[(int64_t)(5 + i) * C + j], rho[i + 3], s3);
-          }
-          for (; i < m; ++i) s0 = fma((double)T[(int64_t)(2 + i) * C + j], rho[i], s0);
-          y[Pn[a]] += (s0 + s1) + (s2 + s3);
-        }
-      }
-      for (int p = tid; p < m; p += NT) {
-        const int q = rowof[n + p];
-        if (q >= 0) y[p] += rho[q];
-      }
+    // The duals decide optimal results only: an unbounded result is certified by a
+    // feasible point and its ray (below), so its dual sweeps and the pass over A are skipped.
+    bool dconv = false;
+    if (status == kStatusOptimal) {
+      // ---- y: rho = c_B - B^T y (one thread per basic structural), y += B~^-T rho (one
+      //      thread per nonbasic slack: its tableau column against rho) ----
+      for (int p = tid; p < m; p += NT) y[p] = colp[p] >= 0 ? (double)T[colp[p]] : 0.0;
       __syncthreads();
-    }
-    // dual feasibility: nonbasic structurals (one pass over A, coalesced across j) and
-    // nonbasic '<=' slacks
-    for (int p = tid; p < m; p += NT) w[p] = sg(p) * y[p];
-    __syncthreads();
-    double dvv = 0;
-    for (int j = tid; j < n; j += NT) {
-      // one column per thread, 16 independent row loads in flight per step
-      double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-      int p = 0;
-      for (; p + 16 <= m; p += 16) {
-        double av[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) av[u] = A[(int64_t)(p + u) * n + j];
-#pragma unroll
-        for (int u = 0; u < 16; u += 4) {
-          s0 = fma(av[u], w[p + u], s0);
-          s1 = fma(av[u + 1], w[p + u + 1], s1);
-          s2 = fma(av[u + 2], w[p + u + 2], s2);
-          s3 = fma(av[u + 3], w[p + u + 3], s3);
+      double ym = 0, rres_d = 0, prev = DBL_MAX;
+      for (int sweep = 0;; ++sweep) {
+        double rl = 0;
+        if (staged) {
+          colsum(AB, y, [&](int a, double sum) {
+            const double val = cS[a] - sum;
+            rho[Srow[a]] = val;
+            rl = fmax(rl, fabs(val));
+          });
+        } else {
+          for (int a = tid; a < k; a += NT) {
+            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            int p = 0;
+            const int v = S[a];
+            for (; p + 4 <= m; p += 4) {
+              s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
+              s1 = fma(sg(p + 1) * A[(int64_t)(p + 1) * n + v], y[p + 1], s1);
+              s2 = fma(sg(p + 2) * A[(int64_t)(p + 2) * n + v], y[p + 2], s2);
+              s3 = fma(sg(p + 3) * A[(int64_t)(p + 3) * n + v], y[p + 3], s3);
+            }
+            for (; p < m; ++p) s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
+            const double val = cS[a] - ((s0 + s1) + (s2 + s3));
+            rho[Srow[a]] = val;
+            rl = fmax(rl, fabs(val));
+          }
         }
+        for (int p = tid; p < m; p += NT) {
+          const int q = rowof[n + p];
+          if (q >= 0) {
+            rho[q] = -y[p];
+            rl = fmax(rl, fabs(y[p]));
+          }
+        }
+        rres_d = rl;
+        ym = 0;
+        for (int p = tid; p < m; p += NT) ym = fmax(ym, fabs(y[p]));
+        bmax2<NT>(rres_d, ym, red);
+        if (sweep == kMaxSweeps || rres_d <= 1e-14 * (1.0 + cm + am * ym) || rres_d > 0.9 * prev) break;
+        prev = rres_d;
+        if (staged) {
+          colsum(TB, rho, [&](int a, double sum) { y[Pn[a]] += sum; });
+        } else {
+          for (int a = tid; a < k; a += NT) {
+            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            int i = 0;
+            const int j = Pcol[a];
+            for (; i + 4 <= m; i += 4) {
+              s0 = fma((double)T[(int64_t)(2 + i) * C + j], rho[i], s0);
+              s1 = fma((double)T[(int64_t)(3 + i) * C + j], rho[i + 1], s1);
+              s2 = fma((double)T[(int64_t)(4 + i) * C + j], rho[i + 2], s2);
+              s3 = fma((double)T[(int64_t)(5 + i) * C + j], rho[i + 3], s3);
+            }
+            for (; i < m; ++i) s0 = fma((double)T[(int64_t)(2 + i) * C + j], rho[i], s0);
+            y[Pn[a]] += (s0 + s1) + (s2 + s3);
+          }
+        }
+        for (int p = tid; p < m; p += NT) {
+          const int q = rowof[n + p];
+          if (q >= 0) y[p] += rho[q];
+        }
+        __syncthreads();
       }
-      for (; p < m; ++p) s0 = fma(A[(int64_t)p * n + j], w[p], s0);
-      if (rowof[j] < 0) dvv = fmax(dvv, c[j] - ((s0 + s1) + (s2 + s3)));
+      // dual feasibility: nonbasic structurals (one pass over A, coalesced across j) and
+      // nonbasic '<=' slacks
+      for (int p = tid; p < m; p += NT) w[p] = sg(p) * y[p];
+      __syncthreads();
+      double dvv = 0;
+      for (int j = tid; j < n; j += NT) {
+        // one column per thread, 16 independent row loads in flight per step
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        int p = 0;
+        for (; p + 16 <= m; p += 16) {
+          double av[16];
+  #pragma unroll
+          for (int u = 0; u < 16; ++u) av[u] = A[(int64_t)(p + u) * n + j];
+  #pragma unroll
+          for (int u = 0; u < 16; u += 4) {
+            s0 = fma(av[u], w[p + u], s0);
+            s1 = fma(av[u + 1], w[p + u + 1], s1);
+            s2 = fma(av[u + 2], w[p + u + 2], s2);
+            s3 = fma(av[u + 3], w[p + u + 3], s3);
+          }
+        }
+        for (; p < m; ++p) s0 = fma(A[(int64_t)p * n + j], w[p], s0);
+        if (rowof[j] < 0) dvv = fmax(dvv, c[j] - ((s0 + s1) + (s2 + s3)));
+      }
+      for (int p = tid; p < m; p += NT)
+        if (colp[p] >= 0 && !is_eq(p)) dvv = fmax(dvv, -y[p]);
+      rd = fmax(bmax<NT>(dvv, red), rres_d) / (1.0 + cm + am * ym);
+      dconv = rres_d <= sq_tol * (1.0 + cm);
     }
-    for (int p = tid; p < m; p += NT)
-      if (colp[p] >= 0 && !is_eq(p)) dvv = fmax(dvv, -y[p]);
-    rd = fmax(bmax<NT>(dvv, red), rres_d) / (1.0 + cm + am * ym);
-    const bool dconv = rres_d <= sq_tol * (1.0 + cm);
 
     // objective of the FP64 basic solution
     double zp = 0;
@@ -598,13 +605,21 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
       zobj = zsum;
       ok = (rp <= tol && rd <= tol && pconv && dconv) ? 1 : 0;
     } else {
-      // ---- unbounded: ray of the entering variable e, d = B^-1 a'_e by refinement ----
+      // ---- unbounded: a feasible point and an improving ray.  The point is the origin
+      //      when it is feasible (b' >= 0 on inequality rows, b = 0 on '=' rows), else the
+      //      final basis, which must then pass the primal test.  The ray of the entering
+      //      variable e, d = B^-1 a'_e by refinement, must keep every basic variable from
+      //      decreasing and gain c_e - c_B^T d > 0. ----
       const int e = out.unb[lp];
       int je = -1;
       for (int j = 1 + tid; j < C; j += NT)
         if (nbv[j] == e) je = j;
       je = (int)bmax<NT>((double)je, red);
-      bool good = e >= 0 && e < n + m && rowof[e] < 0 && je > 0 && rp <= tol && pconv;
+      double onot = 0;
+      for (int p = tid; p < m; p += NT)
+        if (is_eq(p) ? bp[p] != 0.0 : bp[p] < 0.0) onot = 1;
+      const bool origin_ok = bmax<NT>(onot, red) == 0.0;
+      bool good = e >= 0 && e < n + m && rowof[e] < 0 && je > 0 && (origin_ok || (rp <= tol && pconv));
       if (good) {
         for (int i = tid; i < m; i += NT) dv[i] = (double)T[(int64_t)(2 + i) * C + je];
         for (int p = tid; p < m; p += NT)
@@ -619,14 +634,10 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
         }
         viol = bmax<NT>(viol, red);
         double part = 0;
-        if (e < n) {
-          for (int p = tid; p < m; p += NT) part = fma(y[p], sg(p) * A[(int64_t)p * n + e], part);
-        } else {
-          for (int p = tid; p < m; p += NT) if (p == e - n) part = y[p];
-        }
-        const double re = (e < n ? c[e] : 0.0) - bsum<NT>(part, red);
+        for (int a = tid; a < k; a += NT) part = fma(c[S[a]], dv[Srow[a]], part);
+        const double gain = (e < n ? c[e] : 0.0) - bsum<NT>(part, red);
         good = viol <= tol * (1.0 + dmax) && rres2 <= tol * (1.0 + am * dmax) &&
-               re > tol * (1.0 + cm + am * ym) && dconv;
+               gain > tol * (1.0 + cm * (1.0 + dmax));
       }
       ok = good ? 1 : 0;
       rd = -1;

@@ -648,6 +648,7 @@ int blpg_batch_set_warmstart(blpg_ctx* ctx, blpg_batch* B, blpg_ws_model* M) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   }
   B->model = M;
+  warm_route(B);
   return BLPG_OK;
 }
 

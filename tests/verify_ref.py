@@ -101,15 +101,22 @@ def refine_check(A, b, c, sense, status, basis, T, nb, unb_var, amax, tol):
     if status == "optimal":
         return dict(ok=bool(kkt_p <= tol and kkt_d <= tol and pconv and dconv),
                     objective=obj, kkt_p=kkt_p, kkt_d=kkt_d, x=xs)
+    # Unbounded: a feasible point and an improving ray.  The point is the origin when it is
+    # feasible (b' >= 0 on inequality rows, b = 0 on equality rows), else the final basis,
+    # which must then pass the primal test.  The ray d = B^-1 a'_e (refined) must keep every
+    # basic variable from decreasing (d <= 0 in tableau terms, |d| on basic '=' slacks) and
+    # gain c_e - c_B^T d > 0.
+    origin_ok = bool(np.all(np.where(eq, bp == 0, bp >= 0)))
     v = int(unb_var)
     jv = [j for j in range(1, n + 2) if int(nb[j]) == v]
-    if not (0 <= v < n + m) or basic[v] or not jv or kkt_p > tol or not pconv:
+    if not (0 <= v < n + m) or basic[v] or not jv or \
+            (not origin_ok and (kkt_p > tol or not pconv)):
         return dict(ok=False, objective=obj, kkt_p=kkt_p, kkt_d=-1.0, x=xs)
     av = Ap[:, v] if v < n else np.eye(m)[v - n]
     d, rres2, dmax = _refine(T[2:, jv[0]].astype(np.float64), lambda u: av - Bm @ u,
                              lambda r: Bi @ r, 1.0, amax)
     viol = max((abs(d[i]) if (bs[i] >= n and eq[bs[i] - n]) else d[i]) for i in range(m))
-    rv = (c[v] if v < n else 0.0) - float(y @ av)
+    gain = (c[v] if v < n else 0.0) - sum(float(c[bs[i]] * d[i]) for i in range(m) if bs[i] < n)
     good = viol <= tol * (1 + dmax) and rres2 <= tol * (1 + amax * dmax) and \
-        rv > tol * (1 + cm + amax * ym) and dconv
+        gain > tol * (1 + cm * (1 + dmax))
     return dict(ok=bool(good), objective=obj, kkt_p=kkt_p, kkt_d=-1.0, x=xs)
