@@ -280,24 +280,39 @@ def config3_leg(sa, steps, warmup, do_tune=True):
             "fp64_bins_ms": m64["bins_ms"], "tuning": tab}
 
 
-def serving_leg(sa, tune32=None, seconds=2.0, rate=50000.0):
+def serving_leg(sa, tune32=None, seconds=2.0, rate=50000.0, tagged_share=0.1):
     """Serving loop: Poisson arrivals at `rate` LPs/s of single LPs, shapes uniform over
     27x32, 50x48, 74x83, 105x103, 205x203 (generators as the first two workloads, seed 5),
-    FP32 tableau + FP64 check; latency counted from each scheduled arrival."""
+    FP32 tableau + FP64 check; latency counted from each scheduled arrival.  A share of the
+    arrivals are unseen 105x103 instances of 64 parametric families (seed 6), tagged with
+    their family: the server crashes them from a network trained on 256 instances per family
+    (bases from the cold FP64 solver)."""
     shapes = [(27, 32), (50, 48), (74, 83), (105, 103), (205, 203)]
     pools = [sa.gen_config(5, 5, m, n, 512, shape_idx=i, batch_count=1024)
              for i, (m, n) in enumerate(shapes)]
+    nf, fm, fn = 64, 105, 103
+    train, fam_pool = [], []
+    for f in range(nf):
+        train += sa.gen_config(4, 6, fm, fn, 256, shape_idx=f, first=0, batch_count=1024)
+        fam_pool += sa.gen_config(4, 6, fm, fn, 16, shape_idx=f, first=256, batch_count=1024)
+    S64 = sa.Solver(precision="fp64")
+    labels = S64.solve(train)
+    ws = sa.WarmStart(S64, nf, fm, fn, seed=6, steps=1000, lr=3e-3, weight_decay=1e-2)
+    ws.train(train, labels.basis)
     rng = np.random.default_rng(5)
     count = int(rate * seconds)
     which = rng.integers(0, len(shapes), count)
     slot = rng.integers(0, 512, count)
-    lps = [pools[w][k] for w, k in zip(which, slot)]
+    tag = rng.random(count) < tagged_share
+    fslot = rng.integers(0, len(fam_pool), count)
+    lps = [fam_pool[q] if t else pools[w][k] for w, k, t, q in zip(which, slot, tag, fslot)]
     arrival = np.cumsum(rng.exponential(1e6 / rate, count))
     S = sa.Solver(precision="fp32")
     if tune32:
         S.tune_import(tune32)
     srv = sa.Server(S, max_batch=2048, max_wait_us=200, precision="fp32")
-    srv.replay([p[0] for p in pools] * 2, np.arange(10) * 10.0)  # arenas + graphs built
+    srv.set_warmstart(ws)
+    srv.replay([p[0] for p in pools] * 2 + fam_pool[:2], np.arange(12) * 10.0)  # arenas + graphs
     base = srv.stats()
     t0 = time.perf_counter()
     res = srv.replay(lps, arrival)
@@ -305,21 +320,40 @@ def serving_leg(sa, tune32=None, seconds=2.0, rate=50000.0):
     st = srv.stats()
     srv.close()
     S.close()
-    ref = sa.Solver(precision="fp64").solve(lps[:4096])
+    ref = S64.solve(lps[:4096])
+    cold_tag = S64.solve(fam_pool)  # the tagged requests' LPs, solved cold
+    ws.close()
+    S64.close()
     opt = ref.status == sa.OPTIMAL
     rel = (np.abs(res["objective"][:4096][opt] - ref.objective[opt]) /
            np.maximum(1, np.abs(ref.objective[opt])))
     lat = res["latency_us"]
+    cold_piv = cold_tag.pivots[fslot[tag]]
+    tag_ref = cold_tag.objective[fslot[tag]]
+    tag_opt = cold_tag.status[fslot[tag]] == sa.OPTIMAL
+    tag_rel = (np.abs(res["objective"][tag][tag_opt] - tag_ref[tag_opt]) /
+               np.maximum(1, np.abs(tag_ref[tag_opt])))
+    pct = lambda v, q: float(np.percentile(v, q)) if len(v) else 0.0
     return {"workload": f"Poisson {rate:.0f} LPs/s for {seconds} s, shapes uniform over "
-                        "27x32, 50x48, 74x83, 105x103, 205x203, seed 5",
+                        "27x32, 50x48, 74x83, 105x103, 205x203, seed 5; "
+                        f"{tagged_share:.0%} of arrivals family-tagged 105x103 instances "
+                        "(64 families, seed 6) with the learned warm start",
             "offered": count, "completed": int((res["status"] != sa.UNDECIDED).sum()),
             "lps_per_s": count / (float(np.max(arrival + lat)) * 1e-6),
             "span_s": float(np.max(arrival + lat)) * 1e-6, "wall_s": wall,
-            "p50_us": float(np.percentile(lat, 50)), "p99_us": float(np.percentile(lat, 99)),
+            "p50_us": pct(lat, 50), "p99_us": pct(lat, 99),
             "max_us": float(lat.max()), "batches": int(st["batches"] - base["batches"]),
             "mean_batch": (count / max(1, st["batches"] - base["batches"])),
             "status_agreement_fp64": float(np.mean(res["status"][:4096] == ref.status)),
-            "max_rel_obj_err": float(rel.max()) if opt.any() else 0.0}
+            "max_rel_obj_err": float(rel.max()) if opt.any() else 0.0,
+            "tagged": {"count": int(tag.sum()),
+                       "pivots_warm": float(res["pivots"][tag].mean()),
+                       "pivots_cold_fp64": float(cold_piv.mean()),
+                       "p50_us": pct(lat[tag], 50), "p99_us": pct(lat[tag], 99),
+                       "status_agreement_cold_fp64": float(np.mean(
+                           res["status"][tag] == cold_tag.status[fslot[tag]])),
+                       "max_rel_obj_err": float(tag_rel.max()) if tag_opt.any() else 0.0},
+            "untagged": {"p50_us": pct(lat[~tag], 50), "p99_us": pct(lat[~tag], 99)}}
 
 
 def main():

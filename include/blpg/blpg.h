@@ -293,6 +293,14 @@ int blpg_server_submit(blpg_server *server, const blpg_lp *lp, double *x, int32_
 int blpg_server_poll(blpg_server *server, blpg_server_result *out, int32_t max, int32_t wait_us,
                      int32_t *n);
 int blpg_server_stats_get(blpg_server *server, blpg_server_stats *stats);
+/* Learned warm start for family-tagged requests: with `model` attached, a request of the
+ * model's shape whose lp->family is one of the model's families is batched with other
+ * tagged requests of that shape (any mix of families per launch); the model runs inside
+ * those batches' graphs and each LP is crashed from its family's prediction (repaired on
+ * the device like blpg_batch_set_warmstart).  Other requests are unaffected.  Call before
+ * the first tagged request (afterwards: BLPG_E_INVALID); NULL detaches.  The model must
+ * stay alive, and untrained, until the server is destroyed. */
+int blpg_server_set_warmstart(blpg_server *server, blpg_ws_model *model);
 /* Load generator: submits lps[k] at arrival_us[k] (microseconds after the call, ascending)
  * from up to 8 client threads (arrival k on thread k mod P), retrying admission, and waits
  * until all completed; latencies count from the scheduled arrival.  Results per LP go to

@@ -109,7 +109,8 @@ ABI_SYMBOLS = [
     "blpg_ws_destroy", "blpg_ws_train", "blpg_ws_predict", "blpg_batch_set_warmstart",
     "blpg_ws_params", "blpg_ws_norm", "blpg_ws_gradient", "blpg_server_create",
     "blpg_server_destroy", "blpg_server_submit", "blpg_server_poll", "blpg_server_stats_get",
-    "blpg_server_replay", "blpg_solve_shared_A_noalloc", "blpg_noalloc_max_count",
+    "blpg_server_replay", "blpg_server_set_warmstart", "blpg_solve_shared_A_noalloc",
+    "blpg_noalloc_max_count",
 ]
 
 _lib: Optional[C.CDLL] = None
@@ -163,6 +164,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
     lib.blpg_server_submit.argtypes = [C.c_void_p, P(_LP), P(C.c_double), I32, P(C.c_int64)]
     lib.blpg_server_poll.argtypes = [C.c_void_p, P(_ServerResult), C.c_int32, C.c_int32, I32]
     lib.blpg_server_stats_get.argtypes = [C.c_void_p, P(_ServerStats)]
+    lib.blpg_server_set_warmstart.argtypes = [C.c_void_p, C.c_void_p]
     lib.blpg_server_replay.argtypes = [C.c_void_p, P(_LP), P(C.c_double), C.c_int64,
                                        P(_ServerResult)]
     lib.blpg_ws_create.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(_WsConfig),
@@ -615,6 +617,13 @@ class Server:
                         "objective": r.objective, "latency_us": r.latency_us, "x": x,
                         "basis": bs})
         return out
+
+    def set_warmstart(self, model: Optional["WarmStart"]) -> None:
+        """Crash requests tagged with one of `model`'s families (lp.family) from its
+        prediction; call before submitting tagged requests."""
+        self.solver._check(self.solver.lib.blpg_server_set_warmstart(
+            self.h, model.h if model is not None else None))
+        self._model = model  # kept alive while attached
 
     def stats(self) -> dict:
         st = _ServerStats()

@@ -30,6 +30,8 @@ struct WsRows {
   float* Z = nullptr;                                   // logits
   float* raw = nullptr;        // raw (b, c) features (host-supplied rows), or nullptr
   int max_seg = 0;             // largest segment (rows)
+  size_t table_bytes = 0;      // the row tables' range at the front of mem
+  bool arena = false;          // tables rewritten before every launch (serving arenas)
 };
 
 struct Bin {
@@ -147,6 +149,12 @@ void warm_route(blpg_batch* B);
 // warm-start inference of a batch's attached model (warmstart.cpp)
 int ws_run_batch(blpg_ctx* ctx, blpg_batch* B, hipStream_t st);
 void ws_free_rows(WsRows& R);
+// serving arenas: a model attached for up to `cap` LPs of any family mix per launch
+int ws_arena_attach(blpg_ctx* ctx, blpg_batch* B, blpg_ws_model* M, int cap);
+// host image of the row tables for slots 0..k-1 of families fam[0..k-1] (every fam[s] in
+// [0, families)); B->ws.table_bytes bytes, copied to B->ws.mem before the launch
+void ws_arena_tables(const blpg_batch* B, const int32_t* fam, int k, void* image);
+void ws_model_shape(const blpg_ws_model* M, int* families, int* m, int* n);
 
 }  // namespace blpg
 

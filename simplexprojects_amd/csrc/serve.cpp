@@ -14,6 +14,10 @@
 //     max_wait_us, or when no arena of that shape is busy, once every reserved slot has
 //     been copied: one H2D per pool, the count to the device, the arena's graph replayed.
 // Admission: arena bytes + bytes of admitted, unpolled requests <= hbm_budget.
+// Learned warm start: with a model attached (blpg_server_set_warmstart), requests of the
+// model's shape tagged with one of its families form shapes of their own, whose arenas run
+// the model inside their graph; before each launch the worker copies the arena's row tables
+// (which slot is which family) next to the inputs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,6 +45,7 @@ namespace {
 struct Meta {
   int64_t ticket = 0;
   int m = 0, n = 0;
+  int family = -1;  // >= 0: tagged for the attached warm-start model
   double* x = nullptr;
   int32_t* basis = nullptr;
   Clock::time_point t0;
@@ -61,6 +66,7 @@ struct Arena {
   int8_t* h_sense = nullptr; // pinned mirror of the sense pool
   int32_t* h_count = nullptr;
   int32_t* d_count = nullptr;
+  void* h_ws = nullptr;      // pinned image of the warm-start row tables (tagged arenas)
   int64_t pool_elems = 0, spool_elems = 0;
   std::vector<Meta> filled;     // reserved slots 0..filled.size()-1 (not launched yet)
   std::atomic<int> copied{0};   // reserved slots whose data is in h_pool
@@ -73,6 +79,7 @@ struct Arena {
 struct Shape {
   int m = 0, n = 0;
   bool sense = false;
+  bool tagged = false;    // family-tagged requests for the warm-start model
   int fill = -1;          // arena taking new requests (-1: none chosen)
   std::deque<Req> queue;  // overflow, in arrival order
   std::vector<std::unique_ptr<Arena>> arenas;
@@ -85,7 +92,9 @@ struct blpg_server {
   blpg_server_config cfg{};
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::map<std::tuple<int, int, int>, std::unique_ptr<Shape>> shapes;
+  std::map<std::tuple<int, int, int, int>, std::unique_ptr<Shape>> shapes;  // m, n, sense, tagged
+  blpg_ws_model* model = nullptr;  // warm-start model of tagged requests
+  int ws_families = 0, ws_m = 0, ws_n = 0;
   std::deque<blpg_server_result> done;
   std::vector<double> lat;  // all completion latencies
   int64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, batches = 0,
@@ -111,9 +120,10 @@ void destroy_arena(Arena& a) {
   if (a.h_pool) (void)hipHostFree(a.h_pool);
   if (a.h_sense) (void)hipHostFree(a.h_sense);
   if (a.h_count) (void)hipHostFree(a.h_count);
+  if (a.h_ws) (void)hipHostFree(a.h_ws);
   if (a.ctx) blpg_destroy(a.ctx);
   a.ctx = nullptr; a.B = nullptr; a.h_pool = nullptr; a.h_sense = nullptr;
-  a.h_count = nullptr; a.d_count = nullptr;
+  a.h_count = nullptr; a.d_count = nullptr; a.h_ws = nullptr;
 }
 
 // An arena: a device batch of `cap` distinct slots of shape (m, n), uploaded once with
@@ -173,6 +183,13 @@ int make_arena(blpg_server* S, Shape& sh, Arena& a) {
   if (a.h_sense) std::memset(a.h_sense, 0, a.spool_elems);
   a.B->d_count = a.d_count;
   a.bytes = a.B->mem_bytes + a.B->res_bytes;
+  if (sh.tagged) {
+    rc = ws_arena_attach(a.ctx, a.B, S->model, cap);
+    if (rc) { S->err = blpg_last_error(a.ctx); return rc; }
+    if (hipHostMalloc(&a.h_ws, a.B->ws.table_bytes, hipHostMallocDefault) != hipSuccess)
+      return BLPG_E_NOMEM;
+    a.bytes += a.B->ws.bytes + sizeof(int32_t) * (size_t)a.B->basis_elems;
+  }
   return BLPG_OK;
 }
 
@@ -187,9 +204,10 @@ void copy_slot(Arena& a, int slot, const double* A, const double* b, const doubl
   if (a.h_sense && d.s >= 0 && sense) std::memcpy(a.h_sense + d.s, sense, d.m);
 }
 
-// Uploads slots 0..k-1 of the arena and replays its graph (asynchronous).
-int fire(Arena& a, int k) {
+// Uploads slots 0..k-1 of the arena (requests `metas`) and replays its graph (asynchronous).
+int fire(Arena& a, const std::vector<Meta>& metas) {
   blpg_batch* B = a.B;
+  const int k = (int)metas.size();
   const LpDesc& last = B->desc[k - 1];
   const int64_t end = last.c + last.n;
   const int64_t send = (a.h_sense && last.s >= 0) ? last.s + last.m : 0;
@@ -199,6 +217,13 @@ int fire(Arena& a, int k) {
       (send > 0 && hipMemcpyAsync(B->d_spool, a.h_sense, send, hipMemcpyHostToDevice, st) != hipSuccess) ||
       hipMemcpyAsync(a.d_count, a.h_count, sizeof(int32_t), hipMemcpyHostToDevice, st) != hipSuccess)
     return BLPG_E_HIP;
+  if (a.h_ws) {
+    std::vector<int32_t> fam(k);
+    for (int s = 0; s < k; ++s) fam[s] = metas[s].family;
+    ws_arena_tables(B, fam.data(), k, a.h_ws);
+    if (hipMemcpyAsync(B->ws.mem, a.h_ws, B->ws.table_bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+      return BLPG_E_HIP;
+  }
   return blpg_batch_solve(a.ctx, B);
 }
 
@@ -318,7 +343,7 @@ void worker_loop(blpg_server* S) {
                   r.sense.empty() ? nullptr : r.sense.data());
         metas.push_back(r.meta);
       }
-      const int rc = fire(*a, k);
+      const int rc = fire(*a, metas);
       {
         std::lock_guard<std::mutex> lk(S->mu);
         a->inflight = std::move(metas);  // busy stays set: finished by the poll above
@@ -327,7 +352,7 @@ void worker_loop(blpg_server* S) {
     }
     for (Arena* a : ready_fill) {
       const int k = (int)a->inflight.size();
-      record(a, k, fire(*a, k));
+      record(a, k, fire(*a, a->inflight));
     }
     if (!progress) {
       std::unique_lock<std::mutex> lk(S->mu);
@@ -369,6 +394,21 @@ void blpg_server_destroy(blpg_server* S) {
   delete S;
 }
 
+int blpg_server_set_warmstart(blpg_server* S, blpg_ws_model* M) {
+  if (!S) return BLPG_E_INVALID;
+  std::lock_guard<std::mutex> ck(S->create_mu);  // no shape is being created meanwhile
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (auto& kv : S->shapes)
+    if (std::get<3>(kv.first)) {
+      S->err = "blpg_server_set_warmstart: tagged requests were already admitted";
+      return BLPG_E_INVALID;
+    }
+  S->model = M;
+  S->ws_families = S->ws_m = S->ws_n = 0;
+  if (M) ws_model_shape(M, &S->ws_families, &S->ws_m, &S->ws_n);
+  return BLPG_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -388,8 +428,13 @@ int submit_at(blpg_server* S, const blpg_lp* lp, double* x, int32_t* basis, Cloc
   for (int j = 0; j < n; ++j)
     if (!std::isfinite(lp->c[j])) return BLPG_E_INVALID;
   const bool has_sense = lp->sense != nullptr;
-  const auto key = std::make_tuple(m, n, (int)has_sense);
   Meta r;
+  {
+    std::lock_guard<std::mutex> lk0(S->mu);
+    if (S->model && lp->family >= 0 && lp->family < S->ws_families && m == S->ws_m && n == S->ws_n)
+      r.family = lp->family;
+  }
+  const auto key = std::make_tuple(m, n, (int)has_sense, r.family >= 0 ? 1 : 0);
   r.m = m;
   r.n = n;
   r.x = x;
@@ -411,6 +456,7 @@ int submit_at(blpg_server* S, const blpg_lp* lp, double* x, int32_t* basis, Cloc
     sh->m = m;
     sh->n = n;
     sh->sense = has_sense;
+    sh->tagged = r.family >= 0;
     const int64_t arena_need = (int64_t)S->cfg.arenas * S->cfg.max_batch * r.bytes;
     if (S->arena_bytes + S->pending_bytes + arena_need > S->cfg.hbm_budget) {
       ++S->rejected;

@@ -56,6 +56,9 @@ struct TrainBufs {
   float* loss = nullptr;  // steps x nfam
 };
 
+int alloc_rows(blpg_ctx* ctx, const blpg_ws_model* M, WsRows& R, TrainBufs* tb, int64_t raw_rows,
+               int steps);
+
 // Groups `count` rows by family (fam[k] < 0: skipped) into 64-row-padded segments and
 // allocates the device buffers of a forward pass (+ training buffers when tb != nullptr).
 int plan_rows(blpg_ctx* ctx, const blpg_ws_model* M, const std::vector<int32_t>& fam, WsRows& R,
@@ -88,11 +91,31 @@ int plan_rows(blpg_ctx* ctx, const blpg_ws_model* M, const std::vector<int32_t>&
   }
   for (int f = 0; f < M->nfam; ++f)
     for (int r = 0; r < R.seg_rows[f]; ++r) row_fam[R.seg_off[f] + r] = f;
+  int rc = alloc_rows(ctx, M, R, tb, raw_rows, steps);
+  if (rc) return rc;
+  std::vector<int32_t> so(3 * M->nfam);
+  for (int f = 0; f < M->nfam; ++f) {
+    so[f] = R.seg_off[f];
+    so[M->nfam + f] = R.seg_rows[f];
+    so[2 * M->nfam + f] = R.fam_count[f];
+  }
+  HIPCHK(ctx, hipMemcpy(R.row_lp, row_lp.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(R.row_fam, row_fam.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(R.d_seg_off, so.data(), sizeof(int32_t) * so.size(), hipMemcpyHostToDevice));
+  return BLPG_OK;
+}
 
+// Device buffers of a forward pass over R.rows rows (+ training buffers when tb != nullptr).
+// The row tables come first, in one range of R.table_bytes: row_lp [rows], row_fam [rows],
+// then seg_off, seg_rows, fam_count [nfam each], every array 256-byte aligned.
+int alloc_rows(blpg_ctx* ctx, const blpg_ws_model* M, WsRows& R, TrainBufs* tb, int64_t raw_rows,
+               int steps) {
+  const int64_t rows = R.rows;
   size_t off = 0;
   const size_t o_rl = off; off += al256(sizeof(int32_t) * rows);
   const size_t o_rf = off; off += al256(sizeof(int32_t) * rows);
   const size_t o_so = off; off += al256(sizeof(int32_t) * M->nfam * 3);
+  R.table_bytes = off;
   const size_t o_X = off; off += al256(2 * (size_t)rows * M->din_pad);
   const size_t o_H1 = off; off += al256(2 * (size_t)rows * M->H);
   const size_t o_H2 = off; off += al256(2 * (size_t)rows * M->H);
@@ -136,15 +159,6 @@ int plan_rows(blpg_ctx* ctx, const blpg_ws_model* M, const std::vector<int32_t>&
     tb->dA1b = (uint16_t*)(b + o_dA1b);
     tb->loss = (float*)(b + o_loss);
   }
-  std::vector<int32_t> so(3 * M->nfam);
-  for (int f = 0; f < M->nfam; ++f) {
-    so[f] = R.seg_off[f];
-    so[M->nfam + f] = R.seg_rows[f];
-    so[2 * M->nfam + f] = R.fam_count[f];
-  }
-  HIPCHK(ctx, hipMemcpy(R.row_lp, row_lp.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemcpy(R.row_fam, row_fam.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemcpy(R.d_seg_off, so.data(), sizeof(int32_t) * so.size(), hipMemcpyHostToDevice));
   return BLPG_OK;
 }
 
@@ -351,6 +365,75 @@ int ws_run_batch(blpg_ctx* ctx, blpg_batch* B, hipStream_t st) {
 }
 
 void ws_free_rows(WsRows& R) { free_rows(R); }
+
+// Serving arenas.  A launch holds up to `cap` LPs of any mix of families, so the rows are
+// laid out for the worst case (every family's segment padded to 64 rows: at most
+// cap + 63 * min(families, cap) rows) and the forward GEMMs are sized for the largest
+// possible segment (cap rows); tiles beyond a family's rows in this launch exit at once
+// (k_gemm reads the segment sizes from the device).  The row tables are rewritten before
+// every launch, so one captured graph serves every batch.
+int ws_arena_attach(blpg_ctx* ctx, blpg_batch* B, blpg_ws_model* M, int cap) {
+  if (!M || cap < 1) return fail(ctx, BLPG_E_INVALID, "invalid arguments to ws_arena_attach");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  free_rows(B->ws);
+  B->model = nullptr;
+  B->g_gen[0] = B->g_gen[1] = -1;
+  WsRows& R = B->ws;
+  R.nfam = M->nfam;
+  R.rows = up(cap + (kAl - 1) * std::min<int64_t>(M->nfam, cap), kAl);
+  R.max_seg = (int)up(cap, kAl);
+  R.arena = true;
+  int rc = alloc_rows(ctx, M, R, nullptr, 0, 0);
+  if (rc) { free_rows(R); return rc; }
+  // no LP until the first launch's tables arrive
+  HIPCHK(ctx, launch_fill_i32(R.row_lp, -1, R.rows, ctx->stream));
+  HIPCHK(ctx, launch_fill_i32(R.d_seg_off, 0, 3 * (int64_t)M->nfam, ctx->stream));
+  if (!B->d_warm) {
+    if (hipMalloc(&B->warm_mem, sizeof(int32_t) * std::max<int64_t>(B->basis_elems, 1)) != hipSuccess) {
+      free_rows(R);
+      return fail(ctx, BLPG_E_NOMEM, "hipMalloc for warm lists failed");
+    }
+    B->d_warm = (int32_t*)B->warm_mem;
+    HIPCHK(ctx, launch_fill_i32(B->d_warm, -1, B->basis_elems, ctx->stream));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  B->model = M;
+  warm_route(B);
+  return BLPG_OK;
+}
+
+void ws_arena_tables(const blpg_batch* B, const int32_t* fam, int k, void* image) {
+  const WsRows& R = B->ws;
+  const int nf = R.nfam;
+  char* img = (char*)image;
+  int32_t* row_lp = (int32_t*)img;
+  int32_t* row_fam = (int32_t*)(img + al256(sizeof(int32_t) * R.rows));
+  int32_t* so = (int32_t*)(img + 2 * al256(sizeof(int32_t) * R.rows));
+  std::fill(so, so + 3 * nf, 0);
+  int32_t* seg_off = so;
+  int32_t* seg_rows = so + nf;
+  int32_t* count = so + 2 * nf;
+  for (int s = 0; s < k; ++s) count[fam[s]]++;
+  int64_t rows = 0;
+  for (int f = 0; f < nf; ++f) {
+    seg_off[f] = (int32_t)rows;
+    seg_rows[f] = (int32_t)up(count[f], kAl);
+    rows += seg_rows[f];
+  }
+  std::fill(row_lp, row_lp + R.rows, -1);
+  std::fill(row_fam, row_fam + R.rows, 0);
+  for (int f = 0; f < nf; ++f)
+    std::fill(row_fam + seg_off[f], row_fam + seg_off[f] + seg_rows[f], f);
+  std::vector<int32_t> fill(nf, 0);
+  for (int s = 0; s < k; ++s) row_lp[seg_off[fam[s]] + fill[fam[s]]++] = s;
+}
+
+void ws_model_shape(const blpg_ws_model* M, int* families, int* m, int* n) {
+  *families = M->nfam;
+  *m = M->m;
+  *n = M->n;
+}
 
 }  // namespace blpg
 

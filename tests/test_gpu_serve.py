@@ -74,3 +74,50 @@ def test_server_poisson_replay():
     assert (res["latency_us"] > 0).all()
     srv.close()
     S.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_server_tagged_warmstart_50k(precision):
+    """The serving workload at its offered rate (Poisson 50k LPs/s, the five shapes, 64 GB
+    budget) with a share of 105x103 requests tagged to known parametric families: every
+    result is checked against the oracle; untagged FP64 results take the oracle's pivots,
+    tagged ones are crashed from the trained network and take fewer pivots
+    than the oracle's cold solves of the same LPs."""
+    import oracle
+    nf, m, n = 8, 105, 103
+    train, tagged = [], []
+    for f in range(nf):
+        train += sa.gen_config(4, 7, m, n, 256, shape_idx=f, first=0, batch_count=1024)
+        tagged += sa.gen_config(4, 7, m, n, 96, shape_idx=f, first=256, batch_count=1024)
+    S = sa.Solver(precision="fp64")
+    labels = S.solve(train)
+    ws = sa.WarmStart(S, nf, m, n, seed=7, steps=400, lr=3e-3, weight_decay=1e-2)
+    ws.train(train, labels.basis)
+    lps = mixed([1200, 1200, 1200, 1200, 200]) + tagged
+    lps = [lps[k] for k in np.random.default_rng(2).permutation(len(lps))]
+    rng = np.random.default_rng(3)
+    arrival = np.cumsum(rng.exponential(1e6 / 50000.0, len(lps)))
+    Sp = sa.Solver(precision=precision)
+    srv = sa.Server(Sp, max_batch=512, max_wait_us=200, precision=precision)
+    srv.set_warmstart(ws)
+    res = srv.replay(lps, arrival)
+    st = srv.stats()
+    srv.close()
+    ref = oracle.solve_batch(lps, threads=8)
+    is_tag = np.array([lp.family >= 0 for lp in lps])
+    assert is_tag.sum() == nf * 96 and st["completed"] == len(lps) and st["rejected"] == 0
+    assert np.array_equal(res["status"], ref["status"])
+    opt = ref["status"] == sa.OPTIMAL
+    rel = np.abs(res["objective"] - ref["objective"]) / np.maximum(1, np.abs(ref["objective"]))
+    assert rel[opt].max() <= (1e-9 if precision == "fp64" else 1e-6)
+    if precision == "fp64":
+        # (objectives come from the FP64 check's refined solve: within 1e-9, not bitwise)
+        plain = ~is_tag
+        assert np.array_equal(res["pivots"][plain], ref["pivots"][plain])
+    warm_piv = res["pivots"][is_tag].mean()
+    cold_piv = ref["pivots"][is_tag].mean()
+    assert warm_piv < 0.9 * cold_piv, (warm_piv, cold_piv)
+    assert 0 < st["p50_us"] <= st["p99_us"]
+    ws.close()
+    Sp.close()
+    S.close()
