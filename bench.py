@@ -12,8 +12,9 @@ when optimal, whose objective is within 1e-6 * max(1, |obj_fp64|), per second, w
 inputs resident in HBM when the timed region starts.  `end_to_end` times the same steps
 with the inputs' host-to-device copy inside them (the reference's timed ComputeLP: HtoD +
 kernels + DtoH, simplex.cu:944-953): each step repacks the batch into its pinned staging
-(host work, the reference's untimed setConstratint) and then times solve (whose graph copies
-every size bin's inputs just before its kernel) + fetch.
+(host work, the reference's untimed setConstratint) and then times solve (which copies each
+size bin's inputs from pinned staging right before that bin's kernel, on the bin's stream,
+so later bins' copies overlap earlier bins' kernels) + fetch.
 
 The roofline object covers the dominant solve kernel (most algorithmic FLOPs): FLOPs per
 launch (sum over its LPs of pivots * 2*(m+1)*(n+1), one more row per phase-I pivot) over its
@@ -49,6 +50,7 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 FP32_FMA_TFLOPS = 78.6     # one non-packed FMA per lane per clock (the kernels' only form)
 CLOCK_GHZ = 2.4            # MI355X peak engine clock
+N_SIMD = 256 * 4           # MI355X: 256 CUs x 4 SIMDs
 KNAMES = {1: "stream", 2: "lds", 3: "tile", 4: "wave"}
 
 
@@ -175,6 +177,7 @@ def algorithmic_flops(lps, res, idx):
 
 
 def traffic_table():
+    """Per-route counters of the profiled solve kernels (profiles/traffic.json)."""
     try:
         with open(os.path.join(HERE, "profiles", "traffic.json")) as f:
             return json.load(f)
@@ -404,7 +407,13 @@ def main():
     kname = KNAMES[meta32["bins"][bi]["kernel"]]
     kparam = meta32["bins"][bi]["param"]
     route = f"{kname}[{kparam}] fp32 {bin_shape[0]}x{bin_shape[1]} x{len(idx)}"
-    traffic = traffic_table().get(route)
+    prof = traffic_table().get(route) or {}
+    traffic = prof.get("traffic_bytes")
+    # VALU issue fraction: the route's profiled VALU instruction count per launch (fixed by
+    # the pivot sequence) over the issue slots of this launch (one wave64 VALU instruction
+    # per SIMD per 4 clocks, at the profiled clock)
+    valu_frac = (prof["valu_insts"] / (N_SIMD * ms * 1e-3 * prof["clock_ghz"] * 1e9 / 4)
+                 if "valu_insts" in prof else None)
     maxpiv = int(max(r32.pivots[k] for k in idx))
 
     cpu = None
@@ -442,19 +451,23 @@ def main():
                      "algorithmic_flops": flops[bi],
                      "frac_of_fma_rate": achieved / FP32_FMA_TFLOPS,
                      "hbm_frac": (traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+                     "valu_issue_frac": valu_frac,
                      "critical_lp_pivots": maxpiv,
                      "cycles_per_pivot_critical_lp": ms * 1e-3 * CLOCK_GHZ * 1e9 / maxpiv,
                      "note": "latency-bound: the tableau stays in registers; frac is FLOPs "
                              "over the FP32 vector peak (157.3 TFLOP/s); frac_of_fma_rate "
-                             "over the non-packed FMA rate (78.6 TFLOP/s)"},
+                             "over the non-packed FMA rate (78.6 TFLOP/s); valu_issue_frac "
+                             "is the profiled VALU instruction count over the launch's "
+                             "issue slots (1024 SIMDs, one wave64 VALU op per 4 clocks)"},
         "cpu_baseline": cpu,
         "end_to_end": {"lps_per_s": n_good / (e2e["ms_per_step"] * 1e-3),
                        "ms_per_step": e2e["ms_per_step"], "median_ms": e2e["median_ms"],
                        "pack_ms_untimed": e2e["pack_ms"], "same_results": e2e["same_results"],
                        "vs_cpu": ((n_good / (e2e["ms_per_step"] * 1e-3)) / cpu["value"]) if cpu else None,
-                       "note": "HtoD of every bin's inputs inside each step (graph memcpy nodes "
-                               "from pinned staging, overlapped with earlier bins' kernels) + "
-                               "solve + DtoH; the repack into staging is untimed host work"},
+                       "note": "HtoD of every bin's inputs inside each step (async copies "
+                               "from pinned staging, each bin's right before its kernel, "
+                               "overlapped with earlier bins' kernels) + solve + DtoH; the "
+                               "repack into staging is untimed host work"},
         "fp32": {"lps_per_s_all": count / t32, "accurate": n_good, "ms_per_step": t32 * 1e3,
                  "median_step_ms": meta32["median_step_s"] * 1e3,
                  "fallbacks": meta32["fallback_count"], "bins_ms": meta32["bins_ms"],
