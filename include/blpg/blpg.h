@@ -76,8 +76,10 @@ typedef struct {
   int32_t phase1_pivots;
   int32_t fallback_used; /* 1 if the FP32 result failed the FP64 check and FP64 re-solved it */
   double objective;      /* c^T x (FP64 re-solve when verified, else the tableau value) */
-  double kkt_primal;     /* FP64 primal residual of the final basis (-1 if not computed) */
-  double kkt_dual;       /* FP64 dual residual (-1 if not computed) */
+  double kkt_primal;     /* FP64 primal residual of the final basis (-1 if not computed);
+                            FP32 INFEASIBLE: the Farkas certificate's relative violation */
+  double kkt_dual;       /* FP64 dual residual (-1 if not computed); FP32 INFEASIBLE: the
+                            certificate's margin -y^T b' / ((1 + |b|_inf) |y|_inf) */
   double *x;             /* caller-allocated n entries, or NULL */
   int32_t *basis;        /* caller-allocated m entries (variable ids), or NULL */
 } blpg_result;
@@ -95,7 +97,8 @@ typedef struct {
   int32_t kernel;      /* BLPG_KERNEL_* */
   int32_t device;      /* HIP device ordinal */
   int32_t use_graph;   /* 1 = replay the batch through a captured hipGraph (default 1) */
-  int32_t reserved;
+  int32_t sibling_warm; /* blpg_solve_shared_A: 1 = crash LPs 1.. from LP 0's final basis
+                           when LP 0 ends OPTIMAL or UNBOUNDED (default 0: every LP cold) */
 } blpg_options;
 
 void blpg_default_options(blpg_options *opt);
@@ -116,7 +119,11 @@ int blpg_set_options(blpg_ctx *ctx, const blpg_options *opt);
 int blpg_solve_batch(blpg_ctx *ctx, const blpg_lp *lps, int64_t count, blpg_result *out);
 
 /* The reference's batch shape: one A, b (and senses) shared by `count` objectives C
- * (count x n, row-major), GPU_Tester.cu:119-127 + simplex.cu:463-550, 880-890. */
+ * (count x n, row-major), GPU_Tester.cu:119-127 + simplex.cu:463-550, 880-890.
+ * With opt.sibling_warm, LP 0 is solved first; when it ends OPTIMAL or UNBOUNDED its final
+ * basis is primal feasible for every LP of the batch (same A, b, senses), so LPs 1.. are
+ * crashed into it (warm_basis) and skip phase I: their results equal a solve with that
+ * warm list (status, objective; pivots count the crash and the rest from there). */
 int blpg_solve_shared_A(blpg_ctx *ctx, int32_t m, int32_t n, const double *A, const double *b,
                         const int8_t *sense, const double *C, int64_t count, blpg_result *out);
 
@@ -283,6 +290,8 @@ typedef struct {
 } blpg_server_stats;
 
 int blpg_server_create(blpg_ctx *ctx, const blpg_server_config *cfg, blpg_server **server);
+/* Waits for the arenas' work (at most 60 s in all; an arena still running then is not
+ * freed) and releases the server. */
 void blpg_server_destroy(blpg_server *server);
 /* Copies the LP (shape m x n, '<=' rows only or with senses).  x / basis of the result are
  * written to the caller's buffers (n doubles, m int32; may be NULL) before the ticket is
@@ -304,7 +313,9 @@ int blpg_server_set_warmstart(blpg_server *server, blpg_ws_model *model);
 /* Load generator: submits lps[k] at arrival_us[k] (microseconds after the call, ascending)
  * from up to 8 client threads (arrival k on thread k mod P), retrying admission, and waits
  * until all completed; latencies count from the scheduled arrival.  Results per LP go to
- * out[k] (optional). */
+ * out[k] (optional).  With no result for 60 s it stops and returns BLPG_E_HIP, the server's
+ * state in blpg_last_error(NULL) (per calling thread).  A request whose launch fails (or
+ * whose stream reports an error) completes as BLPG_UNDECIDED. */
 int blpg_server_replay(blpg_server *server, const blpg_lp *lps, const double *arrival_us,
                        int64_t count, blpg_server_result *out);
 

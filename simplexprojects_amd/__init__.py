@@ -56,7 +56,7 @@ class _Options(C.Structure):
                 ("verify", C.c_int32), ("eps_d", C.c_double), ("eps_p", C.c_double),
                 ("tol_feas", C.c_double), ("kkt_tol", C.c_double), ("hbm_budget", C.c_int64),
                 ("kernel", C.c_int32), ("device", C.c_int32), ("use_graph", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("sibling_warm", C.c_int32)]
 
 
 class _Info(C.Structure):
@@ -389,7 +389,8 @@ class Solver:
     def __init__(self, precision: str = "fp64", kernel: str = "auto", verify: bool = True,
                  max_iter: int = 0, bland_after: int = 0, eps_d: float = 0.0,
                  eps_p: float = 0.0, tol_feas: float = 0.0, kkt_tol: float = 0.0,
-                 hbm_budget: int = 0, device: int = 0, use_graph: bool = True):
+                 hbm_budget: int = 0, device: int = 0, use_graph: bool = True,
+                 sibling_warm: bool = False):
         self.lib = load_library()
         o = _Options()
         self.lib.blpg_default_options(C.byref(o))
@@ -399,6 +400,7 @@ class Solver:
         o.max_iter, o.bland_after = max_iter, bland_after
         o.eps_d, o.eps_p, o.tol_feas, o.kkt_tol = eps_d, eps_p, tol_feas, kkt_tol
         o.hbm_budget, o.device, o.use_graph = hbm_budget, device, int(use_graph)
+        o.sibling_warm = int(sibling_warm)
         self.opts = o
         h = C.c_void_p()
         rc = self.lib.blpg_create(C.byref(h), C.byref(o))
@@ -636,7 +638,9 @@ class Server:
         arr = (_LP * len(lps))(*[_lp_struct(lp) for lp in lps])
         t = np.ascontiguousarray(arrival_us, dtype=np.float64)
         out = (_ServerResult * len(lps))()
-        self.solver._check(self.solver.lib.blpg_server_replay(self.h, arr, _dp(t), len(lps), out))
+        rc = self.solver.lib.blpg_server_replay(self.h, arr, _dp(t), len(lps), out)
+        if rc != 0:  # the replay's own message (a stall report) is kept per calling thread
+            raise BlpgError(rc, self.solver.lib.blpg_last_error(None).decode())
         dt = np.dtype([("ticket", np.int64), ("status", np.int32), ("pivots", np.int32),
                        ("objective", np.float64), ("latency_us", np.float64)])
         assert dt.itemsize == C.sizeof(_ServerResult)

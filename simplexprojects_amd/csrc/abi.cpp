@@ -1233,7 +1233,19 @@ int blpg_solve_shared_A(blpg_ctx* ctx, int32_t m, int32_t n, const double* A, co
     lps[k].family = -1;
     lps[k].warm_basis = nullptr;
   }
-  return blpg_solve_batch(ctx, lps.data(), count, out);
+  if (!ctx->opt.sibling_warm || count < 2) return blpg_solve_batch(ctx, lps.data(), count, out);
+  // sibling basis: LP 0 first; a basis it ends phase II on is primal feasible for all
+  std::vector<int32_t> pilot((size_t)m);
+  blpg_result r0 = out[0];
+  r0.basis = pilot.data();
+  int rc = blpg_solve_batch(ctx, lps.data(), 1, &r0);
+  if (rc) return rc;
+  if (out[0].basis) std::memcpy(out[0].basis, pilot.data(), sizeof(int32_t) * (size_t)m);
+  r0.basis = out[0].basis;
+  out[0] = r0;
+  if (r0.status == BLPG_OPTIMAL || r0.status == BLPG_UNBOUNDED)
+    for (int64_t k = 1; k < count; ++k) lps[k].warm_basis = pilot.data();
+  return blpg_solve_batch(ctx, lps.data() + 1, count - 1, out + 1);
 }
 
 }  // extern "C"

@@ -24,8 +24,15 @@
 //             entering variable e's ray d = B^-1 a'_e (refined the same way) with
 //             d <= tol (1 + |d|) (|d| on basic '=' slacks), residual <= tol (1 +
 //             |A|_max |d|), and gain c_e - c_B^T d > tol (1 + |c|_inf (1 + |d|_inf)).
-// FP32 results that fail (or FP32 INFEASIBLE whose phase-I objective is not clearly
-// negative, or iteration-limit / numeric statuses) are appended to the fallback list.
+//   INFEASIBLE (FP32, phase I ended with x0 basic) the phase-I duals y, read from row 1 at
+//             the slack columns and refined like y above against the auxiliary problem's
+//             costs (0 everywhere but x0, whose own row is left out), form a Farkas
+//             certificate: y >= 0 on inequality rows (clamped), y^T A'_j >= -tol |A|_max
+//             |y|_inf for every structural j and y^T b' <= -sqrt(tol) (1 + |b|_inf) |y|_inf;
+//             infeasibility found by the equality stage (a row with no usable pivot) has
+//             no such certificate and is re-solved.
+// FP32 results that fail (or have iteration-limit / numeric statuses) are appended to the
+// fallback list.
 //
 // Every sweep touches only the k basic structural columns of A' and the k nonbasic slack
 // columns of the tableau (k = number of basic structurals).  Both m x k blocks are staged
@@ -221,17 +228,21 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   int* Pn = (int*)(wsp + L.Pn);
   int* Pcol = (int*)(wsp + L.Pcol);
 
-  const bool checkable = (status == kStatusOptimal || status == kStatusUnbounded) && out.tab;
-  int k = 0, bad = 0;
+  // an FP32 INFEASIBLE result is certified by the phase-I duals (Farkas), see below
+  const bool infeas = status == kStatusInfeasible && fp32_source;
+  const bool checkable =
+      (status == kStatusOptimal || status == kStatusUnbounded || infeas) && out.tab;
+  int k = 0, kq = 0, r0 = -1, bad = 0;
   if (checkable) {
-    // ---- basis bookkeeping ----
+    // ---- basis bookkeeping (the auxiliary x0, id n+m, is basic at the end of an
+    //      infeasible phase I) ----
     for (int v = tid; v < n + m + 1; v += NT) rowof[v] = -1;
     for (int p = tid; p < m; p += NT) colp[p] = -1;
     __syncthreads();
     for (int i = tid; i < m; i += NT) {
       const int v = basis[i];
       bs[i] = v;
-      if (v < 0 || v >= n + m) bad = 1;
+      if (v < 0 || v >= n + m + (infeas ? 1 : 0)) bad = 1;
       else rowof[v] = i;
     }
     for (int j = 1 + tid; j < C; j += NT) {
@@ -241,11 +252,17 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
     bad = (int)bmax<NT>((double)bad, red);
     k = compact<NT>(n, [&](int j) { return rowof[j] >= 0; }, [&](int j) { return rowof[j]; },
                     S, Srow, red_i);
-    const int kq = compact<NT>(m, [&](int p) { return colp[p] >= 0; },
-                               [&](int p) { return colp[p]; }, Pn, Pcol, red_i);
-    if (k != kq) bad = 1;
+    kq = compact<NT>(m, [&](int p) { return colp[p] >= 0; },
+                     [&](int p) { return colp[p]; }, Pn, Pcol, red_i);
+    if (infeas) {
+      r0 = rowof[n + m];
+      // (infeasibility found by the equality stage leaves x0 out: no certificate here)
+      if (r0 < 0) bad = 1;
+    }
+    if (kq != k + (r0 >= 0 ? 1 : 0)) bad = 1;
   }
-  const bool fits = checkable && !bad && stage_bytes(m, k, (int)sizeof(TT)) <= (size_t)stage_cap;
+  const int kk = kq > k ? kq : k;  // columns of the staged blocks
+  const bool fits = checkable && !bad && stage_bytes(m, kk, (int)sizeof(TT)) <= (size_t)stage_cap;
   if (checkable && !bad && !fits && (defer || NT < 256)) return false;
   // the first-pass kernel only ever runs staged (its global-memory sweeps are not built)
   const bool staged = NT < 256 || fits;
@@ -260,29 +277,33 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   int ok = 2;
   double rp = -1, rd = -1, zobj = out.obj[lp];
   int kx = -1;
-  bool keep_basis = status == kStatusOptimal;  // warm-start a fallback from this basis
+  // warm-start a fallback from this basis (the crash skips x0's id): an optimal one, a
+  // phase-I basis that ended with x0 basic (3-4x fewer FP64 pivots than a cold re-solve
+  // on the two-phase LPs whose certificate does not refine), or (below) an unbounded one
+  // whose basic solution refined
+  bool keep_basis = status == kStatusOptimal || (infeas && r0 >= 0);
   if (!checkable) {
-    if (fp32_source)
-      ok = (status == kStatusInfeasible && out.wfin[lp] < -1e-3 * (1.0 + bm)) ? 1 : 0;
+    if (fp32_source) ok = 0;  // iteration limit, numeric trouble: re-solved in FP64
   } else if (bad) {
     ok = 0;
   } else {
-    const int kpad = k | 1;
+    const int kpad = kk | 1;
     double* AB = (double*)stg;                         // m x kpad: A'[p][S[a]]
     TT* TB = (TT*)(stg + (size_t)m * kpad * 8);        // m x kpad: T[2+i][Pcol[a]]
     for (int p = tid; p < m; p += NT) bp[p] = sg(p) * b[p];
     for (int a = tid; a < k; a += NT) cS[a] = c[S[a]];
-    if (staged && k > 0) {
+    if (staged && kk > 0) {
       // one pass of gathers; thread (tp, ta) copies rows tp, tp + per, ... of column ta,
       // eight rows per step with all loads of a step in flight together.  128 threads: the
       // last step's rows past m re-read row m - 1 and are not stored; 256 threads (whose
       // registers the clamped step would spill): the rows left after the full steps one
-      // at a time.
-      const int per = k >= NT ? 1 : NT / k;
-      const int ta = k >= NT ? tid : tid % k, tp = k >= NT ? 0 : tid / k;
+      // at a time.  (With x0 basic the T block has one column more than the A block; its
+      // A column a = k is a copy of column k-1, or of column 0 of A, never read.)
+      const int per = kk >= NT ? 1 : NT / kk;
+      const int ta = kk >= NT ? tid : tid % kk, tp = kk >= NT ? 0 : tid / kk;
       if (tp < per) {
-        for (int a = ta; a < k; a += NT) {
-          const int ca = S[a], cp = Pcol[a];
+        for (int a = ta; a < kk; a += NT) {
+          const int ca = a < k ? S[a] : (k > 0 ? S[k - 1] : 0), cp = Pcol[a];
           if constexpr (NT < 256) {
             for (int p = tp; p < m; p += 8 * per) {
               double av[8];
@@ -436,15 +457,16 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
       }
       __syncthreads();
     };
-    // column sums over a staged block: emit(a, sum_p M[p][a] v[p]) for a < k.  Thread
-    // (tp, ta) sums rows tp, tp + per, ... of column ta (all NT threads busy when k < NT);
+    // column sums over a staged block: emit(a, sum_p M[p][a] v[p]) for a < cnt.  Thread
+    // (tp, ta) sums rows tp, tp + per, ... of column ta (all NT threads busy when cnt < NT);
     // the per partials of a column are then added in order.
-    auto colsum = [&](const auto* M, const double* v, auto&& emit) {
+    auto colsum = [&](const auto* M, const double* v, int cnt, auto&& emit) {
       const int pc = part_cap(m, NT) / kpad;
-      const int per = k >= NT ? 1 : (NT / k < pc ? NT / k : pc);
-      const int ta = k >= NT ? tid : tid % k, tp = k >= NT ? 0 : tid / k;
-      if (k > 0 && tp < per) {
-        for (int a = ta; a < k; a += NT) {
+      const int per = cnt >= NT || cnt == 0 ? 1 : (NT / cnt < pc ? NT / cnt : pc);
+      const int ta = cnt >= NT || cnt == 0 ? tid : tid % cnt,
+                tp = cnt >= NT || cnt == 0 ? 0 : tid / cnt;
+      if (cnt > 0 && tp < per) {
+        for (int a = ta; a < cnt; a += NT) {
           double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
           int p = tp;
           for (; p + 3 * per < m; p += 4 * per) {
@@ -458,7 +480,7 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
         }
       }
       __syncthreads();
-      for (int a = tid; a < k; a += NT) {
+      for (int a = tid; a < cnt; a += NT) {
         double sum = 0;
         for (int q = 0; q < per; ++q) sum += part[q * kpad + a];
         emit(a, sum);
@@ -479,39 +501,25 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
       }
     };
 
-    // ---- x_B ----
-    for (int i = tid; i < m; i += NT) x[i] = (double)T[(int64_t)(2 + i) * C];
-    __syncthreads();
-    double xm = 0, rres = 0;
-    refine(x, false, -1, 1.0 + bm, rres, xm);
-    double pv = 0;
-    for (int i = tid; i < m; i += NT) {
-      const int v = bs[i];
-      const double xv = x[i];
-      if (v < n) pv = fmax(pv, -xv);
-      else pv = fmax(pv, is_eq(v - n) ? fabs(xv) : -xv);
-    }
-    pv = bmax<NT>(pv, red);
-    rp = fmax(pv, rres) / (1.0 + bm + am * xm);
-    const bool pconv = rres <= sq_tol * (1.0 + bm);
-    keep_basis = keep_basis || pconv;
-
-    // The duals decide optimal results only: an unbounded result is certified by a
-    // feasible point and its ray (below), so its dual sweeps and the pass over A are skipped.
-    bool dconv = false;
-    if (status == kStatusOptimal) {
-      // ---- y: rho = c_B - B^T y (one thread per basic structural), y += B~^-T rho (one
-      //      thread per nonbasic slack: its tableau column against rho) ----
-      for (int p = tid; p < m; p += NT) y[p] = colp[p] >= 0 ? (double)T[colp[p]] : 0.0;
+    if (infeas) {
+      // ---- infeasible: the phase-I duals y (row 1 at the nonbasic slack columns, 0 at
+      //      basic slacks) refined against B^T y = c_B of the auxiliary problem, whose
+      //      costs are 0 but at x0 (x0's column is not kept, so its row's residual is left
+      //      out: the sweeps fix every other row and keep y's scale); then the Farkas test
+      //      in FP64 on y clamped to >= 0 on inequality rows:
+      //        (y^T A')_j >= -tol |A|_max |y|_inf for every structural j,
+      //        y^T b' <= -sqrt(tol) (1 + |b|_inf) |y|_inf,
+      //      which no x >= 0 with A'x <= b' (= on '=' rows) can meet, as
+      //      0 <= y^T A' x <= y^T b' < 0. ----
+      for (int p = tid; p < m; p += NT) y[p] = colp[p] >= 0 ? (double)T[C + colp[p]] : 0.0;
       __syncthreads();
       double ym = 0, rres_d = 0, prev = DBL_MAX;
       for (int sweep = 0;; ++sweep) {
         double rl = 0;
         if (staged) {
-          colsum(AB, y, [&](int a, double sum) {
-            const double val = cS[a] - sum;
-            rho[Srow[a]] = val;
-            rl = fmax(rl, fabs(val));
+          colsum(AB, y, k, [&](int a, double sum) {
+            rho[Srow[a]] = -sum;
+            rl = fmax(rl, fabs(sum));
           });
         } else {
           for (int a = tid; a < k; a += NT) {
@@ -525,9 +533,9 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
               s3 = fma(sg(p + 3) * A[(int64_t)(p + 3) * n + v], y[p + 3], s3);
             }
             for (; p < m; ++p) s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
-            const double val = cS[a] - ((s0 + s1) + (s2 + s3));
-            rho[Srow[a]] = val;
-            rl = fmax(rl, fabs(val));
+            const double sum = (s0 + s1) + (s2 + s3);
+            rho[Srow[a]] = -sum;
+            rl = fmax(rl, fabs(sum));
           }
         }
         for (int p = tid; p < m; p += NT) {
@@ -537,16 +545,17 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
             rl = fmax(rl, fabs(y[p]));
           }
         }
+        if (tid == 0) rho[r0] = 0.0;
         rres_d = rl;
         ym = 0;
         for (int p = tid; p < m; p += NT) ym = fmax(ym, fabs(y[p]));
         bmax2<NT>(rres_d, ym, red);
-        if (sweep == kMaxSweeps || rres_d <= 1e-14 * (1.0 + cm + am * ym) || rres_d > 0.9 * prev) break;
+        if (sweep == kMaxSweeps || rres_d <= 1e-14 * am * ym || rres_d > 0.9 * prev) break;
         prev = rres_d;
         if (staged) {
-          colsum(TB, rho, [&](int a, double sum) { y[Pn[a]] += sum; });
+          colsum(TB, rho, kq, [&](int a, double sum) { y[Pn[a]] += sum; });
         } else {
-          for (int a = tid; a < k; a += NT) {
+          for (int a = tid; a < kq; a += NT) {
             double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
             int i = 0;
             const int j = Pcol[a];
@@ -566,11 +575,15 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
         }
         __syncthreads();
       }
-      // dual feasibility: nonbasic structurals (one pass over A, coalesced across j) and
-      // nonbasic '<=' slacks
-      for (int p = tid; p < m; p += NT) w[p] = sg(p) * y[p];
+      // y clamped to >= 0 on inequality rows (w = sign-applied, so A w = A'^T y per column)
+      double yneg = 0;
+      for (int p = tid; p < m; p += NT) {
+        const double yp = y[p];
+        if (!is_eq(p)) yneg = fmax(yneg, -yp);
+        w[p] = sg(p) * (is_eq(p) ? yp : fmax(yp, 0.0));
+      }
       __syncthreads();
-      double dvv = 0;
+      double vg = 0;
       for (int j = tid; j < n; j += NT) {
         // one column per thread, 16 independent row loads in flight per step
         double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -588,59 +601,180 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
           }
         }
         for (; p < m; ++p) s0 = fma(A[(int64_t)p * n + j], w[p], s0);
-        if (rowof[j] < 0) dvv = fmax(dvv, c[j] - ((s0 + s1) + (s2 + s3)));
+        vg = fmax(vg, -((s0 + s1) + (s2 + s3)));
       }
-      for (int p = tid; p < m; p += NT)
-        if (colp[p] >= 0 && !is_eq(p)) dvv = fmax(dvv, -y[p]);
-      rd = fmax(bmax<NT>(dvv, red), rres_d) / (1.0 + cm + am * ym);
-      dconv = rres_d <= sq_tol * (1.0 + cm);
-    }
-
-    // objective of the FP64 basic solution
-    double zp = 0;
-    for (int a = tid; a < k; a += NT) zp = fma(c[S[a]], x[Srow[a]], zp);
-    const double zsum = bsum<NT>(zp, red);
-    kx = k;
-    if (status == kStatusOptimal) {
-      zobj = zsum;
-      ok = (rp <= tol && rd <= tol && pconv && dconv) ? 1 : 0;
+      double hp = 0;
+      for (int p = tid; p < m; p += NT) hp = fma(b[p], w[p], hp);
+      bmax2<NT>(vg, yneg, red);
+      const double h = bsum<NT>(hp, red);
+      const double gs = am * ym;
+      rp = fmax(vg / fmax(gs, DBL_MIN), yneg / fmax(ym, DBL_MIN));
+      rd = -h / ((1.0 + bm) * fmax(ym, DBL_MIN));
+      ok = (ym > 0 && vg <= tol * gs && yneg <= tol * ym && -h >= sq_tol * (1.0 + bm) * ym) ? 1 : 0;
     } else {
-      // ---- unbounded: a feasible point and an improving ray.  The point is the origin
-      //      when it is feasible (b' >= 0 on inequality rows, b = 0 on '=' rows), else the
-      //      final basis, which must then pass the primal test.  The ray of the entering
-      //      variable e, d = B^-1 a'_e by refinement, must keep every basic variable from
-      //      decreasing and gain c_e - c_B^T d > 0. ----
-      const int e = out.unb[lp];
-      int je = -1;
-      for (int j = 1 + tid; j < C; j += NT)
-        if (nbv[j] == e) je = j;
-      je = (int)bmax<NT>((double)je, red);
-      double onot = 0;
-      for (int p = tid; p < m; p += NT)
-        if (is_eq(p) ? bp[p] != 0.0 : bp[p] < 0.0) onot = 1;
-      const bool origin_ok = bmax<NT>(onot, red) == 0.0;
-      bool good = e >= 0 && e < n + m && rowof[e] < 0 && je > 0 && (origin_ok || (rp <= tol && pconv));
-      if (good) {
-        for (int i = tid; i < m; i += NT) dv[i] = (double)T[(int64_t)(2 + i) * C + je];
-        for (int p = tid; p < m; p += NT)
-          ae[p] = e < n ? sg(p) * A[(int64_t)p * n + e] : (e - n == p ? 1.0 : 0.0);
-        __syncthreads();
-        double dmax = 0, rres2 = 0;
-        refine(dv, true, e, 1.0, rres2, dmax);
-        double viol = 0;
-        for (int i = tid; i < m; i += NT) {
-          const int bv = bs[i];
-          viol = fmax(viol, (bv >= n && is_eq(bv - n)) ? fabs(dv[i]) : dv[i]);
-        }
-        viol = bmax<NT>(viol, red);
-        double part = 0;
-        for (int a = tid; a < k; a += NT) part = fma(c[S[a]], dv[Srow[a]], part);
-        const double gain = (e < n ? c[e] : 0.0) - bsum<NT>(part, red);
-        good = viol <= tol * (1.0 + dmax) && rres2 <= tol * (1.0 + am * dmax) &&
-               gain > tol * (1.0 + cm * (1.0 + dmax));
+      // ---- x_B ----
+      for (int i = tid; i < m; i += NT) x[i] = (double)T[(int64_t)(2 + i) * C];
+      __syncthreads();
+      double xm = 0, rres = 0;
+      refine(x, false, -1, 1.0 + bm, rres, xm);
+      double pv = 0;
+      for (int i = tid; i < m; i += NT) {
+        const int v = bs[i];
+        const double xv = x[i];
+        if (v < n) pv = fmax(pv, -xv);
+        else pv = fmax(pv, is_eq(v - n) ? fabs(xv) : -xv);
       }
-      ok = good ? 1 : 0;
-      rd = -1;
+      pv = bmax<NT>(pv, red);
+      rp = fmax(pv, rres) / (1.0 + bm + am * xm);
+      const bool pconv = rres <= sq_tol * (1.0 + bm);
+      keep_basis = keep_basis || pconv;
+
+      // The duals decide optimal results only: an unbounded result is certified by a
+      // feasible point and its ray (below), so its dual sweeps and the pass over A are skipped.
+      bool dconv = false;
+      if (status == kStatusOptimal) {
+        // ---- y: rho = c_B - B^T y (one thread per basic structural), y += B~^-T rho (one
+        //      thread per nonbasic slack: its tableau column against rho) ----
+        for (int p = tid; p < m; p += NT) y[p] = colp[p] >= 0 ? (double)T[colp[p]] : 0.0;
+        __syncthreads();
+        double ym = 0, rres_d = 0, prev = DBL_MAX;
+        for (int sweep = 0;; ++sweep) {
+          double rl = 0;
+          if (staged) {
+            colsum(AB, y, k, [&](int a, double sum) {
+              const double val = cS[a] - sum;
+              rho[Srow[a]] = val;
+              rl = fmax(rl, fabs(val));
+            });
+          } else {
+            for (int a = tid; a < k; a += NT) {
+              double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+              int p = 0;
+              const int v = S[a];
+              for (; p + 4 <= m; p += 4) {
+                s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
+                s1 = fma(sg(p + 1) * A[(int64_t)(p + 1) * n + v], y[p + 1], s1);
+                s2 = fma(sg(p + 2) * A[(int64_t)(p + 2) * n + v], y[p + 2], s2);
+                s3 = fma(sg(p + 3) * A[(int64_t)(p + 3) * n + v], y[p + 3], s3);
+              }
+              for (; p < m; ++p) s0 = fma(sg(p) * A[(int64_t)p * n + v], y[p], s0);
+              const double val = cS[a] - ((s0 + s1) + (s2 + s3));
+              rho[Srow[a]] = val;
+              rl = fmax(rl, fabs(val));
+            }
+          }
+          for (int p = tid; p < m; p += NT) {
+            const int q = rowof[n + p];
+            if (q >= 0) {
+              rho[q] = -y[p];
+              rl = fmax(rl, fabs(y[p]));
+            }
+          }
+          rres_d = rl;
+          ym = 0;
+          for (int p = tid; p < m; p += NT) ym = fmax(ym, fabs(y[p]));
+          bmax2<NT>(rres_d, ym, red);
+          if (sweep == kMaxSweeps || rres_d <= 1e-14 * (1.0 + cm + am * ym) || rres_d > 0.9 * prev) break;
+          prev = rres_d;
+          if (staged) {
+            colsum(TB, rho, k, [&](int a, double sum) { y[Pn[a]] += sum; });
+          } else {
+            for (int a = tid; a < k; a += NT) {
+              double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+              int i = 0;
+              const int j = Pcol[a];
+              for (; i + 4 <= m; i += 4) {
+                s0 = fma((double)T[(int64_t)(2 + i) * C + j], rho[i], s0);
+                s1 = fma((double)T[(int64_t)(3 + i) * C + j], rho[i + 1], s1);
+                s2 = fma((double)T[(int64_t)(4 + i) * C + j], rho[i + 2], s2);
+                s3 = fma((double)T[(int64_t)(5 + i) * C + j], rho[i + 3], s3);
+              }
+              for (; i < m; ++i) s0 = fma((double)T[(int64_t)(2 + i) * C + j], rho[i], s0);
+              y[Pn[a]] += (s0 + s1) + (s2 + s3);
+            }
+          }
+          for (int p = tid; p < m; p += NT) {
+            const int q = rowof[n + p];
+            if (q >= 0) y[p] += rho[q];
+          }
+          __syncthreads();
+        }
+        // dual feasibility: nonbasic structurals (one pass over A, coalesced across j) and
+        // nonbasic '<=' slacks
+        for (int p = tid; p < m; p += NT) w[p] = sg(p) * y[p];
+        __syncthreads();
+        double dvv = 0;
+        for (int j = tid; j < n; j += NT) {
+          // one column per thread, 16 independent row loads in flight per step
+          double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+          int p = 0;
+          for (; p + 16 <= m; p += 16) {
+            double av[16];
+    #pragma unroll
+            for (int u = 0; u < 16; ++u) av[u] = A[(int64_t)(p + u) * n + j];
+    #pragma unroll
+            for (int u = 0; u < 16; u += 4) {
+              s0 = fma(av[u], w[p + u], s0);
+              s1 = fma(av[u + 1], w[p + u + 1], s1);
+              s2 = fma(av[u + 2], w[p + u + 2], s2);
+              s3 = fma(av[u + 3], w[p + u + 3], s3);
+            }
+          }
+          for (; p < m; ++p) s0 = fma(A[(int64_t)p * n + j], w[p], s0);
+          if (rowof[j] < 0) dvv = fmax(dvv, c[j] - ((s0 + s1) + (s2 + s3)));
+        }
+        for (int p = tid; p < m; p += NT)
+          if (colp[p] >= 0 && !is_eq(p)) dvv = fmax(dvv, -y[p]);
+        rd = fmax(bmax<NT>(dvv, red), rres_d) / (1.0 + cm + am * ym);
+        dconv = rres_d <= sq_tol * (1.0 + cm);
+      }
+
+      // objective of the FP64 basic solution
+      double zp = 0;
+      for (int a = tid; a < k; a += NT) zp = fma(c[S[a]], x[Srow[a]], zp);
+      const double zsum = bsum<NT>(zp, red);
+      kx = k;
+      if (status == kStatusOptimal) {
+        zobj = zsum;
+        ok = (rp <= tol && rd <= tol && pconv && dconv) ? 1 : 0;
+      } else {
+        // ---- unbounded: a feasible point and an improving ray.  The point is the origin
+        //      when it is feasible (b' >= 0 on inequality rows, b = 0 on '=' rows), else the
+        //      final basis, which must then pass the primal test.  The ray of the entering
+        //      variable e, d = B^-1 a'_e by refinement, must keep every basic variable from
+        //      decreasing and gain c_e - c_B^T d > 0. ----
+        const int e = out.unb[lp];
+        int je = -1;
+        for (int j = 1 + tid; j < C; j += NT)
+          if (nbv[j] == e) je = j;
+        je = (int)bmax<NT>((double)je, red);
+        double onot = 0;
+        for (int p = tid; p < m; p += NT)
+          if (is_eq(p) ? bp[p] != 0.0 : bp[p] < 0.0) onot = 1;
+        const bool origin_ok = bmax<NT>(onot, red) == 0.0;
+        bool good = e >= 0 && e < n + m && rowof[e] < 0 && je > 0 && (origin_ok || (rp <= tol && pconv));
+        if (good) {
+          for (int i = tid; i < m; i += NT) dv[i] = (double)T[(int64_t)(2 + i) * C + je];
+          for (int p = tid; p < m; p += NT)
+            ae[p] = e < n ? sg(p) * A[(int64_t)p * n + e] : (e - n == p ? 1.0 : 0.0);
+          __syncthreads();
+          double dmax = 0, rres2 = 0;
+          refine(dv, true, e, 1.0, rres2, dmax);
+          double viol = 0;
+          for (int i = tid; i < m; i += NT) {
+            const int bv = bs[i];
+            viol = fmax(viol, (bv >= n && is_eq(bv - n)) ? fabs(dv[i]) : dv[i]);
+          }
+          viol = bmax<NT>(viol, red);
+          double part = 0;
+          for (int a = tid; a < k; a += NT) part = fma(c[S[a]], dv[Srow[a]], part);
+          const double gain = (e < n ? c[e] : 0.0) - bsum<NT>(part, red);
+          good = viol <= tol * (1.0 + dmax) && rres2 <= tol * (1.0 + am * dmax) &&
+                 gain > tol * (1.0 + cm * (1.0 + dmax));
+        }
+        ok = good ? 1 : 0;
+        rd = -1;
+      }
     }
   }
   __syncthreads();
@@ -663,8 +797,8 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
     }
   }
   // The fallback is warm-started from the FP32 basis when that basis ended phase II as
-  // optimal, or as unbounded with a basic solution that refined (only the ray failed);
-  // other failed results restart cold (crash list cleared).
+  // optimal, phase I with x0 basic, or phase II as unbounded with a basic solution that
+  // refined (only the ray failed); other failed results restart cold (crash list cleared).
   if (fp32_source && ok == 0 && fail_list && !keep_basis) {
     int32_t* bw = out.basis + D.basis;
     for (int i = tid; i < m; i += NT) bw[i] = -1;

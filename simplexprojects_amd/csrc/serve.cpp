@@ -25,11 +25,13 @@
 #include <cmath>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <tuple>
 #include <vector>
@@ -109,6 +111,29 @@ struct blpg_server {
 namespace {
 
 constexpr int64_t kLargeTableau = 150 * 150;  // (m+2)(n+2) from which an arena is masked
+constexpr int kStallSeconds = 60;             // bound of the replay / destroy waits
+
+// The server's state in one line (for a stalled replay's error).
+std::string describe(blpg_server* S) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  std::string out = "err='" + S->err + "'";
+  char buf[160];
+  for (auto& kv : S->shapes) {
+    const Shape& sh = *kv.second;
+    std::snprintf(buf, sizeof buf, "; %dx%d%s%s fill=%d queued=%zu", sh.m, sh.n,
+                  sh.sense ? " senses" : "", sh.tagged ? " tagged" : "", sh.fill,
+                  sh.queue.size());
+    out += buf;
+    for (auto& a : sh.arenas) {
+      std::snprintf(buf, sizeof buf, " [busy=%d failed=%d filled=%zu copied=%d inflight=%zu "
+                    "stream=%s]", a->busy ? 1 : 0, a->failed ? 1 : 0, a->filled.size(),
+                    a->copied.load(), a->inflight.size(),
+                    a->busy ? hipGetErrorName(hipStreamQuery(a->ctx->stream)) : "-");
+      out += buf;
+    }
+  }
+  return out;
+}
 
 int64_t req_bytes(int m, int n, bool sense) {
   return 8 * ((int64_t)m * n + m + n) + (sense ? m : 0) + 12 * (int64_t)(m + n) + 64;
@@ -124,6 +149,13 @@ void destroy_arena(Arena& a) {
   if (a.ctx) blpg_destroy(a.ctx);
   a.ctx = nullptr; a.B = nullptr; a.h_pool = nullptr; a.h_sense = nullptr;
   a.h_count = nullptr; a.d_count = nullptr; a.h_ws = nullptr;
+}
+
+// Records an error of arena creation (which runs without the server lock).
+int note(blpg_server* S, const char* msg, int rc) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->err = msg;
+  return rc;
 }
 
 // An arena: a device batch of `cap` distinct slots of shape (m, n), uploaded once with
@@ -170,7 +202,7 @@ int make_arena(blpg_server* S, Shape& sh, Arena& a) {
     lps[s].warm_basis = nullptr;
   }
   rc = blpg_batch_upload(a.ctx, lps.data(), cap, &a.B);
-  if (rc) { S->err = blpg_last_error(a.ctx); return rc; }
+  if (rc) return note(S, blpg_last_error(a.ctx), rc);
   a.pool_elems = a.B->pool_elems;
   a.spool_elems = a.B->spool_elems;
   if (hipMalloc(&a.d_count, sizeof(int32_t)) != hipSuccess ||
@@ -185,7 +217,7 @@ int make_arena(blpg_server* S, Shape& sh, Arena& a) {
   a.bytes = a.B->mem_bytes + a.B->res_bytes;
   if (sh.tagged) {
     rc = ws_arena_attach(a.ctx, a.B, S->model, cap);
-    if (rc) { S->err = blpg_last_error(a.ctx); return rc; }
+    if (rc) return note(S, blpg_last_error(a.ctx), rc);
     if (hipHostMalloc(&a.h_ws, a.B->ws.table_bytes, hipHostMallocDefault) != hipSuccess)
       return BLPG_E_NOMEM;
     a.bytes += a.B->ws.bytes + sizeof(int32_t) * (size_t)a.B->basis_elems;
@@ -263,13 +295,22 @@ void worker_loop(blpg_server* S) {
   const auto max_wait = std::chrono::microseconds(S->cfg.max_wait_us);
   while (!S->stop.load()) {
     bool progress = false;
-    // completions
+    // completions: a stream that reports an error (not just "not ready") completes its
+    // arena too, as failed, so its requests return (undecided) instead of waiting forever
     std::vector<Arena*> ready;
     {
       std::lock_guard<std::mutex> lk(S->mu);
       for (auto& kv : S->shapes)
-        for (auto& a : kv.second->arenas)
-          if (a->busy && hipStreamQuery(a->ctx->stream) == hipSuccess) ready.push_back(a.get());
+        for (auto& a : kv.second->arenas) {
+          if (!a->busy) continue;
+          const hipError_t q = hipStreamQuery(a->ctx->stream);
+          if (q == hipErrorNotReady) continue;
+          if (q != hipSuccess) {
+            a->failed = true;
+            S->err = std::string("arena stream error: ") + hipGetErrorString(q);
+          }
+          ready.push_back(a.get());
+        }
     }
     for (Arena* a : ready) { finish(S, *a); progress = true; }
     // launches: the filling arena of each shape, and overflow requests (staged here)
@@ -386,10 +427,18 @@ void blpg_server_destroy(blpg_server* S) {
   S->stop.store(true);
   S->cv_work.notify_all();
   if (S->worker.joinable()) S->worker.join();
+  // the arenas' work is waited for, all within one bound: an arena still running after
+  // kStallSeconds is left allocated rather than freed under a running kernel
+  const auto deadline = Clock::now() + std::chrono::seconds(kStallSeconds);
   for (auto& kv : S->shapes)
     for (auto& a : kv.second->arenas) {
-      if (a->ctx) (void)hipStreamSynchronize(a->ctx->stream);
-      destroy_arena(*a);
+      bool idle = !a->ctx;
+      while (!idle) {
+        idle = hipStreamQuery(a->ctx->stream) != hipErrorNotReady;
+        if (idle || Clock::now() > deadline) break;
+        std::this_thread::sleep_for(std::chrono::microseconds(100));
+      }
+      if (idle) destroy_arena(*a);
     }
   delete S;
 }
@@ -582,7 +631,7 @@ int blpg_server_replay(blpg_server* S, const blpg_lp* lps, const double* arrival
     for (int64_t k = t; k < count && !err.load(); k += P) {
       const auto due = start + std::chrono::nanoseconds((int64_t)(arrival_us[k] * 1000.0));
       while (Clock::now() < due) std::this_thread::yield();
-      for (;;) {
+      while (!err.load()) {
         int64_t tk = -1;
         // latency counts from the scheduled arrival
         const int rc = submit_at(S, &lps[k], nullptr, nullptr, due, &tk);
@@ -599,10 +648,21 @@ int blpg_server_replay(blpg_server* S, const blpg_lp* lps, const double* arrival
   for (int t = 0; t < P; ++t) pacers.emplace_back(pace, t);
   std::map<int64_t, blpg_server_result> got;
   std::vector<blpg_server_result> buf(1024);
+  // every wait ends: with no result for kStallSeconds the replay stops and reports what
+  // the server holds (the pacers stop submitting too)
+  auto last = Clock::now();
   while ((int64_t)got.size() < count && !err.load()) {
     int32_t n = 0;
     blpg_server_poll(S, buf.data(), (int32_t)buf.size(), 1000, &n);
     for (int i = 0; i < n; ++i) got[buf[i].ticket] = buf[i];
+    if (n > 0) {
+      last = Clock::now();
+    } else if (Clock::now() - last > std::chrono::seconds(kStallSeconds)) {
+      err.store(BLPG_E_HIP);
+      set_free_error(BLPG_E_HIP, "blpg_server_replay: no result for %d s (%lld of %lld "
+                     "completed): %s", kStallSeconds, (long long)got.size(), (long long)count,
+                     describe(S).c_str());
+    }
   }
   for (auto& th : pacers) th.join();
   if (out)

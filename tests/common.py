@@ -174,3 +174,48 @@ HALL_MCKINNON = (np.array([[0.4, 0.2, -1.4, -0.2], [-7.8, -1.4, 7.8, 0.4]]), np.
 # Beale's example, max form; optimum 5/4.
 BEALE = (np.array([[0.25, -8, -1, 9], [0.5, -12, -0.5, 3], [0, 0, 1, 0]]), np.array([0, 0, 1.0]),
          np.array([0.75, -20, 0.5, -6]))
+
+
+def two_phase_feasible(m, n, count, seed):
+    """Planted-feasible two-phase LPs of the two-phase batch's shapes: A ~ U[-1,1) with 50%
+    of entries zeroed, round(0.3 m) '=' rows and the other rows '<=' / '>=' half and half,
+    built around a planted point x* ~ U[0,2) (b = A x*, plus U[0.1,1) of slack on
+    inequality rows), so b has mixed signs and phase I is needed; the last row
+    sum(x) <= sum(x*) + 5 keeps the LP bounded; c ~ N(0,1).  Every LP is feasible and
+    bounded.  Returns (A, b, c, sense) tuples (numpy PCG64, identical on every host)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        A = rng.uniform(-1.0, 1.0, (m, n))
+        A[rng.random((m, n)) < 0.5] = 0.0
+        x = rng.uniform(0.0, 2.0, n)
+        s = np.zeros(m, np.int8)
+        rows = rng.permutation(m - 1)
+        neq = int(round(0.3 * m))
+        s[rows[:neq]] = 1
+        s[rows[neq:neq + (m - 1 - neq) // 2]] = 2
+        u = rng.uniform(0.1, 1.0, m)
+        b = A @ x + np.where(s == 0, u, np.where(s == 2, -u, 0.0))
+        A[-1] = 1.0
+        b[-1] = x.sum() + 5.0
+        out.append((A, b, rng.normal(size=n), s))
+    return out
+
+
+def load_two_phase():
+    """HiGHS fixture of the planted-feasible two-phase LPs, with the LPs regenerated and
+    checked against the fixture's data hash: [(group, [(A, b, c, sense), ...]), ...]."""
+    import hashlib
+    import json
+    g = json.loads((GOLDEN / "highs_two_phase.json").read_text())
+    out = []
+    for grp in g["groups"]:
+        lps = two_phase_feasible(grp["m"], grp["n"], grp["count"], grp["seed"])
+        h = hashlib.sha256()
+        for A, b, c, s in lps:
+            for a in (A, b, c):
+                h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+            h.update(s.tobytes())
+        assert h.hexdigest() == grp["sha256"], "two-phase generator changed"
+        out.append((grp, lps))
+    return out

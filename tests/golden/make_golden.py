@@ -14,6 +14,10 @@ box only reads the committed outputs).
                       HiGHS status / objective for seeded subsets of the synthetic
                       configurations, with a SHA-256 of the generated data so that the
                       generator is pinned too.
+  highs_two_phase.json
+                      HiGHS optima of planted-feasible two-phase LPs at 174x142 and
+                      205x203 (tests/common.py two_phase_feasible: mixed senses and
+                      mixed-sign b around a planted point), with a SHA-256 of the data.
 
 HiGHS is scipy.optimize.linprog(method="highs") (scipy 1.15.3 in this image).  When it
 reports "unknown" the model is re-solved with highs-ds, highs-ipm and as a pure
@@ -157,6 +161,34 @@ def make_configs():
             json.dumps({"config": cfg, "seed": spec["seed"], "groups": entries}, indent=0))
 
 
+TWO_PHASE = {"seed": 17, "shapes": [(174, 142), (205, 203)], "count": 12}
+
+
+def make_two_phase():
+    from tests.common import two_phase_feasible
+    groups = []
+    for si, (m, n) in enumerate(TWO_PHASE["shapes"]):
+        lps = two_phase_feasible(m, n, TWO_PHASE["count"], TWO_PHASE["seed"] + si)
+        h = hashlib.sha256()
+        st, ob = [], []
+        for A, b, c, s in lps:
+            for a in (A, b, c):
+                h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+            h.update(s.tobytes())
+            r = highs(A, b, c, s)
+            st.append(r[0])
+            ob.append(r[1])
+        groups.append({"m": m, "n": n, "seed": TWO_PHASE["seed"] + si, "count": len(lps),
+                       "sha256": h.hexdigest(), "status": st, "objective": ob})
+        print("two-phase", m, n, {k: st.count(k) for k in set(st)}, flush=True)
+    (HERE / "highs_two_phase.json").write_text(json.dumps({"groups": groups}, indent=0))
+
+
 if __name__ == "__main__":
-    make_netlib()
-    make_configs()
+    # `make_golden.py two_phase` writes only highs_two_phase.json (no reference files read)
+    if sys.argv[1:] == ["two_phase"]:
+        make_two_phase()
+    else:
+        make_netlib()
+        make_configs()
+        make_two_phase()

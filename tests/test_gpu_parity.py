@@ -215,3 +215,62 @@ def test_shared_A_matches_oracle():
         assert rs.objective[k] == o["objective"], k
         assert np.array_equal(rs.basis[k], o["basis"]), k
     S.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_two_phase_feasible_golden(precision):
+    """Planted-feasible two-phase LPs at 174x142 and 205x203 (phase I, then phase II to the
+    optimum): status and objective equal HiGHS's (1e-9 relative FP64, 1e-6 after the FP64
+    check in FP32); FP64 runs take the oracle's pivots exactly, FP32 runs the oracle's FP32
+    replay's pivots unless re-solved by the fallback."""
+    from tests.common import load_two_phase
+    groups = load_two_phase()
+    lps = [sa.LP(A, b, c, s) for _, grp_lps in groups for (A, b, c, s) in grp_lps]
+    status = np.concatenate([grp["status"] for grp, _ in groups])
+    obj = np.concatenate([grp["objective"] for grp, _ in groups])
+    S = sa.Solver(precision=precision)
+    r = S.solve(lps)
+    assert np.array_equal(r.status, status)
+    err = np.abs(r.objective - obj) / np.maximum(1.0, np.abs(obj))
+    assert err.max() <= (1e-9 if precision == "fp64" else 1e-6)
+    assert (r.phase1_pivots > 0).all()
+    for k, lp in enumerate(lps):
+        o = oracle.solve(lp.A, lp.b, lp.c, lp.sense, precision=precision)
+        assert r.pivots[k] == o["pivots"] or (precision == "fp32" and r.fallback[k]), k
+    S.close()
+
+
+def test_shared_A_sibling_warm_matches_oracle():
+    """One A, b and senses (a planted-feasible two-phase LP, 174x142) with 40 objectives:
+    copies of one objective, as the reference's own shared-A batch (GPU_Tester.cu:119-127),
+    then that objective perturbed by 5% per entry.  With sibling_warm, LP 0 is solved cold
+    and LPs 1.. are crashed from its final basis, primal feasible for all of them.  Every
+    LP is bit-identical to the oracle's replay with the same warm list (status, pivots,
+    objective, basis), ends at the cold solve's optimum, and skips phase I: about a fifth of
+    the cold pivots."""
+    from tests.common import load_two_phase
+    grp, lps = load_two_phase()[0]
+    A, b, c0, s = lps[0]
+    rng = np.random.default_rng(5)
+    Cm = np.tile(c0, (40, 1))
+    Cm[8:] *= 1.0 + 0.05 * rng.normal(size=(32, len(c0)))
+    S = sa.Solver(verify=False, sibling_warm=True)
+    r = S.solve_shared_A(A, b, Cm, s)
+    o0 = oracle.solve(A, b, Cm[0], s)
+    assert o0["status"] == oracle.OPTIMAL
+    assert r.status[0] == o0["status"] and r.pivots[0] == o0["pivots"]
+    assert np.array_equal(r.basis[0], o0["basis"])
+    warm_piv, cold_piv = [], []
+    for k in range(1, len(Cm)):
+        o = oracle.solve(A, b, Cm[k], s, warm=o0["basis"])
+        assert r.status[k] == o["status"] and r.pivots[k] == o["pivots"], k
+        assert r.objective[k] == o["objective"], k
+        assert np.array_equal(r.basis[k], o["basis"]), k
+        oc = oracle.solve(A, b, Cm[k], s)
+        assert oc["status"] == o["status"] == oracle.OPTIMAL, k
+        assert rel_err(o["objective"], oc["objective"]) <= 1e-9, k
+        assert r.phase1_pivots[k] < oc["phase1_pivots"], k
+        warm_piv.append(o["pivots"])
+        cold_piv.append(oc["pivots"])
+    assert np.mean(warm_piv) < 0.35 * np.mean(cold_piv), (np.mean(warm_piv), np.mean(cold_piv))
+    S.close()

@@ -79,3 +79,36 @@ def test_large_lps_global_sweeps_match_restatement():
     r = S.solve(lps)
     assert compare(lps, r) >= 8
     S.close()
+
+
+def test_two_phase_infeasible_certified_on_device():
+    """FP32 INFEASIBLE results of the two-phase LPs (HiGHS: infeasible) are accepted by the
+    device's Farkas check exactly where the restatement's certificate is clear, with the
+    same violation and margin; a result the restatement cannot certify is re-solved."""
+    from tests.common import load_highs
+    g = load_highs(3)
+    lps = []
+    for grp in g["groups"]:
+        lps += sa.gen_config(3, g["seed"], grp["m"], grp["n"], grp["count"],
+                             shape_idx=grp["shape_idx"], first=grp["first"],
+                             batch_count=grp["batch_count"])
+    status = np.concatenate([np.asarray(grp["status"]) for grp in g["groups"]])
+    S = sa.Solver(precision="fp32")
+    r = S.solve(lps)
+    assert np.array_equal(r.status, status)
+    decided = 0
+    for k, lp in enumerate(lps):
+        o = oracle.solve(lp.A, lp.b, lp.c, lp.sense, precision="fp32", tab=True)
+        assert o["status"] == oracle.INFEASIBLE
+        ref = refine_check(lp.A, lp.b, lp.c, lp.sense, "infeasible", o["basis"], o["tab"],
+                           o["nb"], -1, float(np.abs(lp.A).max()), TOL)
+        clear_yes = ref["ok"] and ref["kkt_p"] < 0.1 * TOL and ref["kkt_d"] > 10 * np.sqrt(TOL)
+        if clear_yes:
+            decided += 1
+            assert not r.fallback[k], k
+            assert abs(r.kkt_dual[k] - ref["kkt_d"]) <= 1e-6 * ref["kkt_d"], k
+            assert r.kkt_primal[k] <= 0.1 * TOL, k
+        elif not ref["ok"]:
+            assert r.fallback[k], k
+    assert decided >= len(lps) // 2
+    S.close()

@@ -244,3 +244,15 @@ def test_dual_repair_after_rhs_change():
         crash = int((r0["basis"] < lp.n).sum())
         dual_used += w["phase1_pivots"] > crash
     assert dual_used >= 3
+
+
+def test_two_phase_feasible_matches_highs():
+    """Planted-feasible two-phase LPs (mixed senses, mixed-sign b, 174x142 and 205x203): the
+    oracle's FP64 replay runs phase I and ends at HiGHS's optimum on every one."""
+    from tests.common import load_two_phase
+    for grp, lps in load_two_phase():
+        for k, (A, b, c, s) in enumerate(lps):
+            r = oracle.solve(A, b, c, s)
+            assert grp["status"][k] == oracle.OPTIMAL and r["status"] == oracle.OPTIMAL, k
+            assert rel_err(r["objective"], grp["objective"][k]) <= 1e-9, k
+            assert r["phase1_pivots"] > 0, k

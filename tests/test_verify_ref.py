@@ -9,7 +9,7 @@ import pytest
 import oracle
 import simplexprojects_amd as sa
 from tests.common import highs, load_highs, load_netlib, rel_err
-from tests.verify_ref import refine_check
+from tests.verify_ref import farkas_test, refine_check
 
 TOL = 1e-9
 STATUS = {oracle.OPTIMAL: "optimal", oracle.UNBOUNDED: "unbounded"}
@@ -130,3 +130,72 @@ def test_planted_mistakes_fail():
         if nonb:
             bs[i] = nonb[0]
             assert not check(lp.A, lp.b, lp.c, lp.sense, dict(o, basis=bs))["ok"]
+
+
+def _signed(A, b, s):
+    sg = np.where(np.asarray(s) == 2, -1.0, 1.0)
+    return A * sg[:, None], b * sg, np.asarray(s) == 1
+
+
+def test_farkas_test_textbook():
+    """x1 + x2 <= 1 and x1 + x2 >= 2: y = (1, 1) gives 0 <= -1; every other candidate
+    below misses one condition."""
+    A = np.array([[1.0, 1.0], [1.0, 1.0]])
+    Ap, bp, eq = _signed(A, np.array([1.0, 2.0]), [0, 2])
+    assert farkas_test(Ap, bp, eq, [1.0, 1.0], 1.0, TOL)[0]
+    assert farkas_test(Ap, bp, eq, [3.0, 3.0], 1.0, TOL)[0]  # any positive scale
+    assert not farkas_test(Ap, bp, eq, [1.0, 0.0], 1.0, TOL)[0]   # y^T b' = 1 > 0
+    assert not farkas_test(Ap, bp, eq, [1.0, 0.5], 1.0, TOL)[0]   # y^T A' = (0.5, 0.5), y^T b' = 0
+    assert not farkas_test(Ap, bp, eq, [0.5, 1.0], 1.0, TOL)[0]   # y^T A' = (-0.5, -0.5)
+    assert not farkas_test(Ap, bp, eq, [-1.0, -1.0], 1.0, TOL)[0]  # clamped to 0
+    assert not farkas_test(Ap, bp, eq, [1.0, 1.0 + 1e-6], 1.0, TOL)[0]  # y^T A' = -1e-6
+    # '=' rows take either sign: x1 - x2 = 1 and x2 - x1 = 1 sum to 0 = 2
+    A = np.array([[1.0, -1.0], [-1.0, 1.0]])
+    Ap, bp, eq = _signed(A, np.array([1.0, 1.0]), [1, 1])
+    assert farkas_test(Ap, bp, eq, [-1.0, -1.0], 1.0, TOL)[0]
+    assert not farkas_test(Ap, bp, eq, [1.0, 1.0], 1.0, TOL)[0]
+
+
+def test_farkas_never_certifies_a_feasible_lp():
+    """Farkas' alternative: a feasible system has no certificate, so no candidate passes
+    (random ones, and ones shaped like the real certificates: nonnegative on inequality
+    rows)."""
+    rng = np.random.default_rng(3)
+    for A, b, c, s in mixed_sense_lps(6):
+        Ap, bp, eq = _signed(A, b, s)
+        am = float(np.abs(A).max())
+        for _ in range(200):
+            y = rng.normal(size=len(b))
+            y[~eq] = np.abs(y[~eq]) * (rng.random(int((~eq).sum())) < 0.5)
+            assert not farkas_test(Ap, bp, eq, y, am, TOL)[0]
+
+
+def test_config3_infeasible_results_certified():
+    """The FP32 oracle's INFEASIBLE results on the two-phase LPs (HiGHS: infeasible) carry
+    a Farkas certificate in their phase-I duals once refined in FP64; a planted mistake
+    (row 1 negated, or the x0 row's basic variable taken for a slack) fails."""
+    g = load_highs(3)
+    certified = 0
+    for grp in g["groups"]:
+        lps = sa.gen_config(3, g["seed"], grp["m"], grp["n"], 4, shape_idx=grp["shape_idx"],
+                            first=grp["first"], batch_count=grp["batch_count"])
+        for k, lp in enumerate(lps):
+            assert grp["status"][k] == oracle.INFEASIBLE
+            o = oracle.solve(lp.A, lp.b, lp.c, lp.sense, precision="fp32", tab=True)
+            assert o["status"] == oracle.INFEASIBLE
+            am = float(np.abs(lp.A).max())
+            args = (lp.A, lp.b, lp.c, lp.sense, "infeasible")
+            r = refine_check(*args, o["basis"], o["tab"], o["nb"], -1, am, TOL)
+            if r["ok"]:
+                certified += 1
+                Ap, bp, eq = _signed(lp.A, lp.b, lp.sense)
+                yc = np.where(eq, r["y"], np.maximum(r["y"], 0))
+                assert float(bp @ yc) < 0 and (Ap.T @ yc).min() >= -TOL * am * np.abs(yc).max()
+            bad = o["tab"].copy()
+            bad[1] *= -1
+            assert not refine_check(*args, o["basis"], bad, o["nb"], -1, am, TOL)["ok"]
+            bs = o["basis"].copy()
+            m, n = lp.A.shape
+            bs[bs == n + m] = n  # x0's row read as the first constraint's slack
+            assert not refine_check(*args, bs, o["tab"], o["nb"], -1, am, TOL)["ok"]
+    assert certified >= 6

@@ -40,9 +40,61 @@ def _refine(v, resid, correct, scale0, amax):
         sweep += 1
 
 
+def farkas_test(Ap, bp, eq, y, amax, tol):
+    """FP64 Farkas test of a candidate y for {x >= 0, A'x <= b' ('=' on eq rows)}, with
+    A', b' the sign-applied rows: y is clamped to >= 0 on inequality rows, then the LP is
+    certified infeasible when (y^T A')_j >= -tol |A|_max |y|_inf for every column j and
+    y^T b' <= -sqrt(tol) (1 + |b|_inf) |y|_inf (0 <= y^T A' x <= y^T b' < 0 for any
+    feasible x).  Returns (ok, violation, margin), both relative to |y|_inf."""
+    y = np.asarray(y, np.float64)
+    ym = np.abs(y).max(initial=0.0)
+    yneg = max(0.0, float((-y[~eq]).max(initial=0.0)))
+    yc = np.where(eq, y, np.maximum(y, 0.0))
+    vg = max(0.0, float((-(Ap.T @ yc)).max(initial=0.0)))
+    h = float(bp @ yc)
+    bm = np.abs(bp).max(initial=0.0)
+    tiny = np.finfo(np.float64).tiny
+    ok = bool(ym > 0 and vg <= tol * amax * ym and yneg <= tol * ym and
+              -h >= np.sqrt(tol) * (1 + bm) * ym)
+    return ok, max(vg / max(amax * ym, tiny), yneg / max(ym, tiny)), -h / ((1 + bm) * max(ym, tiny))
+
+
+def _infeasible_check(Ap, bp, eq, bs, T, nb, amax, tol):
+    """Phase-I duals of an FP32 INFEASIBLE result (x0, id n+m, basic): row 1 of the
+    tableau at the nonbasic slack columns, refined against B^T y = c_B of the auxiliary
+    problem (0 but at x0, whose row is left out), then the Farkas test."""
+    m, n = Ap.shape
+    fail = dict(ok=False, objective=0.0, kkt_p=-1.0, kkt_d=-1.0, x=None)
+    if np.any(bs < 0) or np.any(bs > n + m) or int((bs == n + m).sum()) != 1:
+        return fail
+    colp = -np.ones(m, np.int64)
+    for j in range(1, n + 2):
+        v = int(nb[j])
+        if n <= v < n + m:
+            colp[v - n] = j
+    if (colp >= 0).sum() != (bs < n).sum() + 1:
+        return fail
+    Bi = _binv(T, bs, colp, n, m)
+
+    def resid(v):
+        r = np.zeros(m)
+        for i, bv in enumerate(bs):
+            if bv < n:
+                r[i] = -float(Ap[:, bv] @ v)
+            elif bv < n + m:
+                r[i] = -v[bv - n]
+        return r  # (x0's row: 0)
+    y0 = np.array([float(T[1, colp[p]]) if colp[p] >= 0 else 0.0 for p in range(m)])
+    y, _, _ = _refine(y0, resid, lambda r: Bi.T @ r, 0.0, amax)
+    ok, viol, margin = farkas_test(Ap, bp, eq, y, amax, tol)
+    return dict(ok=ok, objective=0.0, kkt_p=viol, kkt_d=margin, x=None, y=y)
+
+
 def refine_check(A, b, c, sense, status, basis, T, nb, unb_var, amax, tol):
-    """Returns dict(ok, objective, kkt_p, kkt_d, x) for an OPTIMAL or UNBOUNDED result;
-    ok is None when the basis is not a proper slack/structural basis."""
+    """Returns dict(ok, objective, kkt_p, kkt_d, x) for an OPTIMAL or UNBOUNDED result
+    (ok is None when the basis is not a proper slack/structural basis), and for an FP32
+    INFEASIBLE one (phase I ended with x0 basic; ok False without x0) dict(ok, kkt_p =
+    relative Farkas violation, kkt_d = relative margin of y^T b' < 0, y)."""
     A = np.asarray(A, np.float64)
     m, n = A.shape
     sg = np.ones(m) if sense is None else np.where(np.asarray(sense) == 2, -1.0, 1.0)
@@ -51,6 +103,8 @@ def refine_check(A, b, c, sense, status, basis, T, nb, unb_var, amax, tol):
     bp = np.asarray(b, np.float64) * sg
     c = np.asarray(c, np.float64)
     bs = np.asarray(basis, np.int64)
+    if status == "infeasible":
+        return _infeasible_check(Ap, bp, eq, bs, T, nb, amax, tol)
     if np.any(bs < 0) or np.any(bs >= n + m):
         return dict(ok=None)
     colp = -np.ones(m, np.int64)
