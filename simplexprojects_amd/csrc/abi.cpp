@@ -90,6 +90,69 @@ hipError_t upload_warm(blpg_batch* B, const blpg_lp* lps) {
   return hipMemcpy(B->d_warm, w.data(), sizeof(int32_t) * w.size(), hipMemcpyHostToDevice);
 }
 
+// Packs the staging pool's ranges for the copy of the next solve (see blpg_batch::ZRange).
+// Without room for the packed buffers every range is copied dense.
+void pack_ranges(blpg_batch* B) {
+  auto a8 = [](size_t x) { return (x + 7) & ~(size_t)7; };
+  auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  std::vector<std::pair<int64_t, int64_t>> rng;
+  rng.push_back({0, B->common_elems});
+  for (const Bin& b : B->bins) rng.push_back({b.pool_lo, b.pool_hi});
+  size_t cap = 0;
+  for (const auto& r : rng) {
+    const int64_t n = r.second - r.first, nw = (n + 63) / 64;
+    cap += a256(8 * (size_t)nw + a8(4 * (size_t)nw) + 8 * (size_t)n);
+  }
+  B->zr.clear();
+  if (cap > B->z_cap) {
+    if (B->h_z) (void)hipHostFree(B->h_z);
+    if (B->d_z) (void)hipFree(B->d_z);
+    B->h_z = nullptr;
+    B->d_z = nullptr;
+    B->z_cap = 0;
+    if (hipHostMalloc((void**)&B->h_z, cap, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&B->d_z, cap) != hipSuccess) {
+      if (B->h_z) (void)hipHostFree(B->h_z);
+      B->h_z = nullptr;
+      B->d_z = nullptr;
+      return;
+    }
+    B->z_cap = cap;
+  }
+  const double* pool = (const double*)(B->h_stage + B->o_pool);
+  size_t off = 0;
+  for (const auto& r : rng) {
+    blpg_batch::ZRange z;
+    z.lo = r.first;
+    z.hi = r.second;
+    z.off = off;
+    const int64_t n = z.hi - z.lo, nw = (n + 63) / 64;
+    uint64_t* mask = (uint64_t*)(B->h_z + off);
+    int32_t* pref = (int32_t*)(B->h_z + off + 8 * nw);
+    double* vals = (double*)(B->h_z + off + 8 * nw + a8(4 * (size_t)nw));
+    int64_t nnz = 0;
+    for (int64_t w = 0; w < nw && n < INT32_MAX; ++w) {
+      uint64_t mw = 0;
+      pref[w] = (int32_t)nnz;
+      const int64_t e = std::min<int64_t>(64, n - 64 * w);
+      for (int64_t u = 0; u < e; ++u) {
+        const double v = pool[z.lo + 64 * w + u];
+        uint64_t bits;
+        std::memcpy(&bits, &v, 8);
+        if (bits != 0) {  // -0.0 is kept as a value: only +0.0 is implied
+          mw |= 1ull << u;
+          vals[nnz++] = v;
+        }
+      }
+      mask[w] = mw;
+    }
+    z.bytes = 8 * (size_t)nw + a8(4 * (size_t)nw) + 8 * (size_t)nnz;
+    z.packed = n > 0 && n < INT32_MAX && z.bytes * 10 <= (size_t)n * 8 * 9;
+    off += a256(8 * (size_t)nw + a8(4 * (size_t)nw) + 8 * (size_t)n);
+    B->zr.push_back(z);
+  }
+}
+
 }  // namespace
 
 // message of the last failed call that takes no context, per host thread
@@ -477,6 +540,8 @@ void blpg_batch_free(blpg_ctx* ctx, blpg_batch* B) {
     if (B->copied[i]) (void)hipEventDestroy(B->copied[i]);
   }
   if (B->h_stage) (void)hipHostFree(B->h_stage);
+  if (B->h_z) (void)hipHostFree(B->h_z);
+  if (B->d_z) (void)hipFree(B->d_z);
   ws_free_rows(B->ws);
   if (B->h_res) (void)hipHostFree(B->h_res);
   if (B->warm_mem) (void)hipFree(B->warm_mem);
@@ -680,18 +745,40 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing, bool copy) {
   // launch order, one copy after the other -- unless the warm-start network, which reads
   // every family LP's b and c, runs first
   double* hpool = copy ? (double*)(B->h_stage + B->o_pool) : nullptr;
-  auto copy_range = [&](int64_t lo, int64_t hi, hipStream_t cs) -> hipError_t {
-    if (hi <= lo) return hipSuccess;
-    return hipMemcpyAsync(B->d_pool + lo, hpool + lo, sizeof(double) * (hi - lo),
-                          hipMemcpyHostToDevice, cs);
+  // range r (0: the shared arrays, 1 + i: bin i) to the device: packed (the copy, then its
+  // expansion into the pool on the same stream) or dense
+  // (`done`, if given, is recorded once the host-to-device copy itself is enqueued, so the
+  // next copy need not wait for the expansion)
+  auto copy_range = [&](int r, int64_t lo, int64_t hi, hipStream_t cs,
+                        hipEvent_t done = nullptr) -> hipError_t {
+    hipError_t e = hipSuccess;
+    if (hi > lo && r < (int)B->zr.size() && B->zr[r].packed) {
+      const blpg_batch::ZRange& z = B->zr[r];
+      const int64_t nw = (z.hi - z.lo + 63) / 64;
+      const size_t o_pref = 8 * (size_t)nw, o_vals = o_pref + ((4 * (size_t)nw + 7) & ~(size_t)7);
+      e = hipMemcpyAsync(B->d_z + z.off, B->h_z + z.off, z.bytes, hipMemcpyHostToDevice, cs);
+      if (e == hipSuccess && done) e = hipEventRecord(done, cs);
+      if (e == hipSuccess)
+        e = launch_expand_f64((const uint64_t*)(B->d_z + z.off),
+                              (const int32_t*)(B->d_z + z.off + o_pref),
+                              (const double*)(B->d_z + z.off + o_vals), z.hi - z.lo,
+                              B->d_pool + z.lo, cs);
+      return e;
+    }
+    if (hi > lo)
+      e = hipMemcpyAsync(B->d_pool + lo, hpool + lo, sizeof(double) * (hi - lo),
+                         hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess && done) e = hipEventRecord(done, cs);
+    return e;
   };
   if (copy) {
-    HIPCHK(ctx, copy_range(0, B->common_elems, st));
+    HIPCHK(ctx, copy_range(0, 0, B->common_elems, st));
     if (B->spool_elems > 0)
       HIPCHK(ctx, hipMemcpyAsync(B->d_spool, B->h_stage + B->o_spool, (size_t)B->spool_elems,
                                  hipMemcpyHostToDevice, st));
     if (B->model)
-      for (const Bin& b : B->bins) HIPCHK(ctx, copy_range(b.pool_lo, b.pool_hi, st));
+      for (size_t i = 0; i < B->bins.size(); ++i)
+        HIPCHK(ctx, copy_range(1 + (int)i, B->bins[i].pool_lo, B->bins[i].pool_hi, st));
   }
   const bool bin_copies = copy && !B->model;
   int prev_copy = -1;
@@ -737,8 +824,7 @@ static int enqueue_solve(blpg_ctx* ctx, blpg_batch* B, bool timing, bool copy) {
     used[lane_of[i]] = true;
     if (bin_copies) {
       if (prev_copy >= 0) HIPCHK(ctx, hipStreamWaitEvent(s, B->copied[prev_copy], 0));
-      HIPCHK(ctx, copy_range(b.pool_lo, b.pool_hi, s));
-      HIPCHK(ctx, hipEventRecord(B->copied[i], s));
+      HIPCHK(ctx, copy_range(1 + i, b.pool_lo, b.pool_hi, s, B->copied[i]));
       prev_copy = i;
     }
     if (timing) HIPCHK(ctx, hipEventRecord(B->ev[1 + 4 * i], s));
@@ -907,6 +993,7 @@ int blpg_batch_repack(blpg_ctx* ctx, blpg_batch* B, const blpg_lp* lps, int64_t 
   // the previous solve's copies read the staging: wait for them
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   stage_inputs(B, lps);
+  pack_ranges(B);
   if (B->d_warm) {
     const hipError_t e = upload_warm(B, lps);
     if (e != hipSuccess) return fail(ctx, BLPG_E_HIP, "warm lists: %s", hipGetErrorString(e));

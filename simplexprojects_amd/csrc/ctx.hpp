@@ -82,6 +82,20 @@ struct blpg_batch {
   char* h_stage = nullptr;   // pinned host image of desc | pool | sense pool | int32 init
   size_t stage_bytes = 0, o_pool = 0, o_spool = 0;
   bool dirty = false;        // staging newer than the device copy: the next solve copies it
+  // The copy after a repack sends each pool range (the shared arrays, then each bin's)
+  // packed where that is at least 10% smaller: a bitmask of the elements that are not +0.0,
+  // a running set-bit count per 64-element word and the packed values, expanded on the
+  // device before the range's first reader.  zr[0] is the shared range, zr[1 + i] bin i's.
+  struct ZRange {
+    int64_t lo = 0, hi = 0;   // pool elements
+    size_t off = 0;           // in h_z / d_z: mask [nw] u64, pref [nw] i32 (8-aligned), vals
+    size_t bytes = 0;         // bytes to copy
+    bool packed = false;      // else copied dense from the staging pool
+  };
+  std::vector<ZRange> zr;
+  char* h_z = nullptr;        // pinned
+  char* d_z = nullptr;
+  size_t z_cap = 0;
   char* d_tab = nullptr;  // final-tableau dumps (read by the FP64 check)
   int mmax = 0, nmax = 0;
   void* mem = nullptr;

@@ -108,5 +108,9 @@ size_t verify_ws_stride(int mcap, int ncap);  // 0: vectors in LDS
 
 // small helpers (kernel_util.hip)
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
+// out[0, n) from a packed range (bitmask of the elements that are not +0.0, running set-bit
+// count per 64-element word, the packed values)
+hipError_t launch_expand_f64(const uint64_t* mask, const int32_t* pref, const double* vals,
+                             int64_t n, double* out, hipStream_t st);
 
 }  // namespace blpg

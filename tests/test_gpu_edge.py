@@ -213,3 +213,36 @@ def test_repack_copies_new_inputs(precision):
         batch.repack(b[:10])  # a different LP count
     batch.free()
     S.close()
+
+
+def test_repack_packed_copy_bit_exact():
+    """After a repack the copy sends each pool range packed (bitmask of the entries that
+    are not +0.0, the other values packed) when that is smaller: entries of -0.0 are
+    values and stay -0.0, a bin without zeros goes dense; every result equals the solve of
+    a fresh upload of the same LPs bit for bit."""
+    rng = np.random.default_rng(9)
+    base = sa.gen_config(2, 2, 50, 48, 32, shape_idx=0, batch_count=1024)
+    new = []
+    for lp in sa.gen_config(2, 8, 50, 48, 32, shape_idx=0, batch_count=1024):
+        A = lp.A.copy()
+        A[(A == 0) & (rng.random(A.shape) < 0.5)] = -0.0
+        new.append(sa.LP(A, lp.b.copy(), lp.c.copy()))
+    assert any(np.signbit(lp.A[lp.A == 0]).any() for lp in new)
+    dense = [sa.LP(rng.uniform(0.5, 1.0, (74, 83)), rng.uniform(1, 10, 74),
+                   rng.uniform(0, 1, 83)) for _ in range(16)]
+    for precision in ("fp64", "fp32"):
+        S = sa.Solver(precision=precision)
+        batch = S.upload(base + dense)
+        batch.solve()
+        batch.fetch()
+        batch.repack(new + dense)
+        batch.solve()
+        r = batch.fetch()
+        direct = S.solve(new + dense)
+        assert np.array_equal(r.status, direct.status)
+        assert np.array_equal(r.pivots, direct.pivots)
+        assert np.array_equal(r.objective, direct.objective)
+        for k in range(len(new) + len(dense)):
+            assert np.array_equal(r.x[k], direct.x[k]), k
+        batch.free()
+        S.close()
