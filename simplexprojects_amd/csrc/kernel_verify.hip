@@ -236,12 +236,24 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   if (checkable) {
     // ---- basis bookkeeping (the auxiliary x0, id n+m, is basic at the end of an
     //      infeasible phase I) ----
+    int kl = 0, xl = 0;
+    for (int i = tid; i < m; i += NT) {
+      const int v = basis[i];
+      bs[i] = v;
+      kl += (v >= 0 && v < n) ? 1 : 0;
+      xl |= v == n + m ? 1 : 0;
+    }
+    if (defer || NT < 256) {
+      // an LP this tier hands on is decided from its basis alone (the size of its staged
+      // blocks), before the rest of the bookkeeping
+      const int ke = (int)bsum<NT>((double)kl, red) + ((infeas && bmax<NT>((double)xl, red) > 0) ? 1 : 0);
+      if (stage_bytes(m, ke, (int)sizeof(TT)) > (size_t)stage_cap) return false;
+    }
     for (int v = tid; v < n + m + 1; v += NT) rowof[v] = -1;
     for (int p = tid; p < m; p += NT) colp[p] = -1;
     __syncthreads();
     for (int i = tid; i < m; i += NT) {
-      const int v = basis[i];
-      bs[i] = v;
+      const int v = bs[i];
       if (v < 0 || v >= n + m + (infeas ? 1 : 0)) bad = 1;
       else rowof[v] = i;
     }
