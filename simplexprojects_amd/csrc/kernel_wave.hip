@@ -85,6 +85,28 @@ __device__ __forceinline__ double wlane_read(double x, int l) {
 
 template <typename T>
 __device__ __forceinline__ T wave_fmin(T x) {
+  if constexpr (sizeof(T) == 4) {
+    // FP32: each step is one v_min_f32 with a DPP source (the same IEEE minimum as fminf
+    // on the finite / infinite keys that reach it, without a separate move and the NaN
+    // canonicalisation fminf adds); rows outside a step's row mask keep their value.
+    // s_nop 1: the two wait states a DPP read of a VGPR needs after the VALU write.
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_min_f32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_min_f32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_min_f32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_min_f32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_min_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_min_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+        "s_nop 1"
+        : "+v"(x));
+    return wlane_read(x, 63);
+  }
   x = tmin(x, dpp_mv<kDppXor1, 0xF>(x));
   x = tmin(x, dpp_mv<kDppXor2, 0xF>(x));
   x = tmin(x, dpp_mv<kDppHalfMirror, 0xF>(x));
