@@ -415,7 +415,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   const size_t o_desc = off; off += al(sizeof(LpDesc) * n);
   const size_t o_pool = off; off += al(sizeof(double) * B->pool_elems);
   const size_t o_spool = off; off += al(B->spool_elems + 16);
-  const size_t o_i32 = off; off += al(sizeof(int32_t) * n * 11);    // status pivots p1 unb ok fb all fail idx defer defer2
+  const size_t o_i32 = off; off += al(sizeof(int32_t) * n * 12);    // status pivots p1 unb ok fb all fail idx defer defer2 lastpiv
   const size_t o_f64 = off; off += al(sizeof(double) * n * 6);      // obj wfin chkobj kktp kktd amax
   const size_t o_x = off; off += al(sizeof(double) * B->x_elems);
   const size_t o_basis = off; off += al(sizeof(int32_t) * B->basis_elems);
@@ -451,6 +451,7 @@ int blpg_batch_upload(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, blpg_bat
   B->d_fail = i32 + 7 * n;
   B->d_defer = i32 + 9 * n;
   B->d_defer2 = i32 + 10 * n;
+  B->out.lastpiv = i32 + 11 * n;
   int32_t* d_binidx = i32 + 8 * n;
   double* f64 = (double*)(base + o_f64);
   B->out.obj = f64;

@@ -26,6 +26,8 @@ struct Out {
   double* x;        // pool
   int32_t* basis;   // pool
   double* amax;     // max |A_ij| of the LP's input (written by the solve kernels at set-up)
+  int32_t* lastpiv; // the last pivot, row | column << 16 (tableau indices), -1 when there is
+                    // none or it cannot be undone from the final state; nullptr: not kept
   char* tab;        // final tableaux (row-major, solve precision) + nonbasic ids, for the
                     // FP64 check; nullptr: not dumped
 };

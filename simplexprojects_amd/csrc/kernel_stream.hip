@@ -279,7 +279,7 @@ struct StreamTab {
   }
 
   __device__ __forceinline__ void write_outputs(int lp, const LpDesc& d, const Out& out, int status, int it,
-                                int p1, int unb, T wfin) {
+                                int p1, int unb, T wfin, int last) {
     __syncthreads();
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -290,6 +290,7 @@ struct StreamTab {
       out.pivots[lp] = it;
       out.p1[lp] = p1;
       out.unb[lp] = unb;
+      if (out.lastpiv) out.lastpiv[lp] = last;
       out.obj[lp] = (double)at(0, 0);
       out.wfin[lp] = (double)wfin;
     }
@@ -371,10 +372,10 @@ __global__ __launch_bounds__(NT) void k_stream(const LpDesc* __restrict__ desc,
     t.init(pool + d.a, pool + d.b, pool + d.c, sense);
     const int32_t* warm = warm_pool ? warm_pool + d.basis : nullptr;
     if (!t.set_target(warm)) warm = nullptr;
-    int status, it, p1, unb;
+    int status, it, p1, unb, last;
     T wfin;
-    run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, warm);
-    t.write_outputs(lp, d, out, status, it, p1, unb, wfin);
+    run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, last, warm);
+    t.write_outputs(lp, d, out, status, it, p1, unb, wfin, last);
   }
 }
 

@@ -706,7 +706,7 @@ struct TileTab {
 
   // column 0 is mirrored in rhsbuf (objective T[0][0], basic values T[i][0])
   __device__ __forceinline__ void write_outputs(int lp, const LpDesc& d, const Out& out, int status, int it,
-                                int p1, int unb, T wfin) {
+                                int p1, int unb, T wfin, int last) {
     __syncthreads();
     if (tid == 0) {
       double am = s_am[0];
@@ -716,6 +716,7 @@ struct TileTab {
       out.pivots[lp] = it;
       out.p1[lp] = p1;
       out.unb[lp] = unb;
+      if (out.lastpiv) out.lastpiv[lp] = last;
       out.obj[lp] = (double)rhsbuf[0];
       out.wfin[lp] = (double)wfin;
     }
@@ -793,10 +794,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) 
     t.init(pool + d.a, pool + d.b, pool + d.c, sense);
     const int32_t* warm = warm_pool ? warm_pool + d.basis : nullptr;
     if (!t.set_target(warm)) warm = nullptr;
-    int status, it, p1, unb;
+    int status, it, p1, unb, last;
     T wfin;
-    run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, warm);
-    t.write_outputs(lp, d, out, status, it, p1, unb, wfin);
+    run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, last, warm);
+    t.write_outputs(lp, d, out, status, it, p1, unb, wfin, last);
   }
 }
 

@@ -294,6 +294,7 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   // on the two-phase LPs whose certificate does not refine), or (below) an unbounded one
   // whose basic solution refined
   bool keep_basis = status == kStatusOptimal || (infeas && r0 >= 0);
+  bool undo_last = false;  // the final basis did not refine: warm-start before its last pivot
   if (!checkable) {
     if (fp32_source) ok = 0;  // iteration limit, numeric trouble: re-solved in FP64
   } else if (bad) {
@@ -640,6 +641,7 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
       rp = fmax(pv, rres) / (1.0 + bm + am * xm);
       const bool pconv = rres <= sq_tol * (1.0 + bm);
       keep_basis = keep_basis || pconv;
+      undo_last = !pconv;
 
       // The duals decide optimal results only: an unbounded result is certified by a
       // feasible point and its ray (below), so its dual sweeps and the pass over A are skipped.
@@ -810,10 +812,22 @@ __device__ bool verify_lp(int lp, const LpDesc& D, const double* __restrict__ po
   }
   // The fallback is warm-started from the FP32 basis when that basis ended phase II as
   // optimal, phase I with x0 basic, or phase II as unbounded with a basic solution that
-  // refined (only the ray failed); other failed results restart cold (crash list cleared).
-  if (fp32_source && ok == 0 && fail_list && !keep_basis) {
+  // refined (only the ray failed).  A final basis whose basic solution does not refine
+  // (numerically singular in FP64) is replaced by the basis before the FP32 solve's last
+  // pivot, when that pivot is known: the usual cause is that pivot itself, taken on an
+  // element that is tiny in exact arithmetic.  Its leaving variable is the final nonbasic
+  // variable of the pivot column, and goes back to the pivot row.  Other failed results
+  // restart cold (crash list cleared).
+  if (fp32_source && ok == 0 && fail_list) {
     int32_t* bw = out.basis + D.basis;
-    for (int i = tid; i < m; i += NT) bw[i] = -1;
+    const int lpv = (undo_last && out.lastpiv) ? out.lastpiv[lp] : -1;
+    const int pr = lpv & 0xFFFF, pc = lpv >> 16;
+    const int lvar = (lpv >= 0 && pr >= 2 && pr < R && pc >= 1 && pc < C) ? nbv[pc] : -1;
+    if (lvar >= 0 && lvar < n + m) {
+      if (tid == 0) bw[pr - 2] = lvar;
+    } else if (!keep_basis) {
+      for (int i = tid; i < m; i += NT) bw[i] = -1;
+    }
   }
   return true;
 }

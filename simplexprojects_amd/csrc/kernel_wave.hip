@@ -699,7 +699,7 @@ struct WaveTab {
   __device__ __forceinline__ int crash_prepare(int&, const int8_t*, T) { return -1; }
 
   __device__ __forceinline__ void write_outputs(int lp, const LpDesc& d, const Out& out, int status, int it,
-                                                int p1, int unb, T wfin) {
+                                                int p1, int unb, T wfin, int last) {
     __syncthreads();
     const T obj = rhs_get(0);
     if (tid == 0) {
@@ -710,6 +710,7 @@ struct WaveTab {
       out.pivots[lp] = it;
       out.p1[lp] = p1;
       out.unb[lp] = unb;
+      if (out.lastpiv) out.lastpiv[lp] = last;
       out.obj[lp] = (double)obj;
       out.wfin[lp] = (double)wfin;
     }
@@ -795,10 +796,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) 
     t.init(pool + d.a, pool + d.b, pool + d.c, sense);
     const int32_t* warm = warm_pool ? warm_pool + d.basis : nullptr;
     if (!t.set_target(warm)) warm = nullptr;
-    int status, it, p1, unb;
+    int status, it, p1, unb, last;
     T wfin;
-    run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, warm);
-    t.write_outputs(lp, d, out, status, it, p1, unb, wfin);
+    run_simplex<T>(t, d.m, d.n, sense, o, status, it, p1, unb, wfin, last, warm);
+    t.write_outputs(lp, d, out, status, it, p1, unb, wfin, last);
   }
 }
 

@@ -279,13 +279,18 @@ enum : int {
 // negative rhs below -tol; entering column: smallest max(T[0][j],0) / -T[r][j] over
 // T[r][j] < -eps_p, dual-degenerate columns by the largest |T[r][j]|) restore primal
 // feasibility; then the auxiliary phase I (if still needed) and phase II finish.
+// `last`: the final pivot (row | column << 16), so that a caller can undo it: the variable
+// that left at that row is the final nonbasic variable of that column.
 template <typename T, class Tab>
 __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* sense,
                                             const Opt<T>& o, int& status, int& it, int& p1,
-                                            int& unb, T& wfin, const int32_t* warm = nullptr) {
+                                            int& unb, T& wfin, int& last,
+                                            const int32_t* warm = nullptr) {
   const int maxit = o.max_iter > 0 ? o.max_iter : 50 * (m + n) + 1000;
   const int C = n + 2;
-  it = 0; p1 = 0; status = kStatusOptimal; unb = -1; wfin = T(0);
+  it = 0; p1 = 0; status = kStatusOptimal; unb = -1; wfin = T(0); last = -1;
+  // the last pivot is kept as row | column << 16 where both fit
+  const bool keep_last = m + 2 < 65536 && C < 32768;
   int stage = warm ? kStCrash : (sense ? kStEq : kStAux);
   int ei = 0, deg = 0, ci = 0;
   int bt = 0, bnp = 0;  // block crash: next pivot of the prepared block, its pivots
@@ -418,8 +423,12 @@ __device__ __forceinline__ void run_simplex(Tab& t, int m, int n, const int8_t* 
       t.pivot(pr, ps);
       ++it;
       if (count_p1) ++p1;
+      last = keep_last ? pr | (ps << 16) : -1;
     }
-    if (kill_after) t.kill_col(ps);
+    if (kill_after) {
+      t.kill_col(ps);
+      last = -1;  // a killed column's pivot is not undone
+    }
   }
 }
 

@@ -112,3 +112,33 @@ def test_two_phase_infeasible_certified_on_device():
             assert r.fallback[k], k
     assert decided >= len(lps) // 2
     S.close()
+
+
+def test_singular_final_basis_falls_back_from_before_its_last_pivot():
+    # The 105x103 LPs of the mixed-size batch (seed 2).  LP 388's FP32 solve ends UNBOUNDED
+    # on a basis that is singular in FP64 (cond ~5e7; its last pivot was on an element that
+    # is tiny in exact arithmetic), so its basic solution does not refine and the FP64
+    # fallback starts from the basis before that last pivot: the oracle's FP32 replay
+    # stopped one pivot short.  The device result is the oracle's FP64 replay warm from that
+    # basis, pivot for pivot, in under half the pivots of a cold re-solve.  LP 211's final
+    # basis refines (only its ray fails), so it is re-solved warm from that basis.
+    lps = sa.gen_config(2, 2, 105, 103, 1024, shape_idx=3, batch_count=1024)
+    S = sa.Solver(precision="fp32")
+    r = S.solve(lps)
+    S.close()
+    fb = [k for k in range(len(lps)) if r.fallback[k]]
+    assert 388 in fb and 211 in fb, fb
+    for k, undo in ((388, True), (211, False)):
+        lp = lps[k]
+        o32 = oracle.solve(lp.A, lp.b, lp.c, precision="fp32")
+        start = o32["basis"]
+        if undo:
+            start = oracle.solve(lp.A, lp.b, lp.c, precision="fp32",
+                                 max_iter=o32["pivots"] - 1)["basis"]
+        want = oracle.solve(lp.A, lp.b, lp.c, warm=start)
+        assert r.status[k] == want["status"], k
+        assert r.pivots[k] == want["pivots"], (k, r.pivots[k], want["pivots"])
+        cold = oracle.solve(lp.A, lp.b, lp.c)
+        assert r.status[k] == cold["status"], k
+        if undo:
+            assert r.pivots[k] < 0.5 * cold["pivots"], (r.pivots[k], cold["pivots"])
