@@ -51,7 +51,7 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 FP32_FMA_TFLOPS = 78.6     # one non-packed FMA per lane per clock (the kernels' only form)
 CLOCK_GHZ = 2.4            # MI355X peak engine clock
 N_SIMD = 256 * 4           # MI355X: 256 CUs x 4 SIMDs
-KNAMES = {1: "stream", 2: "lds", 3: "tile", 4: "wave"}
+KNAMES = {0: "check", 1: "stream", 2: "lds", 3: "tile", 4: "wave"}
 
 
 def cpu_model() -> str:

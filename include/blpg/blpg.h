@@ -204,9 +204,12 @@ int blpg_batch_info_get(blpg_ctx *ctx, const blpg_batch *batch, blpg_batch_info 
  * routes later batches of that shape and precision to the fastest; the default route's
  * tile variant is kept unless another instance is more than 3% faster (closer timings are
  * run-to-run noise, and a route that flips moves shapes between bins).  The choice changes
- * speed only: all instances compute bit-identical results.  blpg_tune_set pins a route by
- * hand (param: tile variant, or threads per workgroup); export / import carry the table as
- * text lines "precision m n kernel param ms". */
+ * speed only: all instances compute bit-identical results.  With the FP64 check on, it also
+ * times the shape's check with and without its first LDS tier (31 KB per LP) where a later
+ * tier exists, and skips the first tier when that is more than 3% faster (workloads whose
+ * final bases never fit it); decisions are the same either way.  blpg_tune_set pins a route
+ * by hand (param: tile variant, or threads per workgroup); export / import carry the table as
+ * text lines "precision m n kernel param ms" (kernel 0: the check plan, param 1 = skip). */
 int blpg_autotune(blpg_ctx *ctx, const blpg_lp *lps, int64_t count, int32_t reps);
 int blpg_tune_set(blpg_ctx *ctx, int32_t precision, int32_t m, int32_t n, int32_t kernel,
                   int32_t param);

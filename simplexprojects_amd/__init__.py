@@ -447,8 +447,9 @@ class Solver:
         return Batch(self, lps)
 
     def autotune(self, lps: Sequence[LP], reps: int = 3) -> list[tuple]:
-        """Measure every kernel instance per LP shape and route to the fastest; returns
-        the tuning table as (precision, m, n, kernel, param, ms) tuples."""
+        """Measure every kernel instance per LP shape and route to the fastest, and the
+        shape's check plan (kernel 0 rows: param 1 skips the first check tier); returns the
+        tuning table as (precision, m, n, kernel, param, ms) tuples."""
         lps = list(lps)
         arr = (_LP * len(lps))(*[_lp_struct(lp) for lp in lps])
         self._check(self.lib.blpg_autotune(self.ctx, arr, len(lps), reps))

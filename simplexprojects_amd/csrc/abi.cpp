@@ -664,8 +664,9 @@ static int plan_solve(blpg_ctx* ctx, blpg_batch* B, SolvePlan& P) {
       // tiers of growing LDS per LP; the last one used takes whatever the others defer
       const int tsz = fp32 ? 4 : 8;
       std::vector<int>& tv = P.vtiers[i];
+      const bool skip0 = (ctx->tuner.check_flags(precision, mc, nc) & 1) != 0;
       for (int tier = 0; tier < 3; ++tier) {
-        if (!verify_tier_used(tier, mc, nc, tsz)) continue;
+        if (!verify_tier_used(tier, mc, nc, tsz) || (tier == 0 && skip0)) continue;
         tv.push_back(tier);
         tv.push_back(persistent_grid(ctx, verify_max_blocks_per_cu(tier, mc, nc), cnt));
         if (tier < 2 && verify_tier_covers(tier, mc, nc, tsz)) break;
@@ -1231,6 +1232,40 @@ int blpg_tune_import(blpg_ctx* ctx, const char* text) {
   return BLPG_OK;
 }
 
+// The check plan of one shape: when its check starts with the first tier (31 KB per LP) and
+// has a later one, time the check (eager, HIP events) with and without the first tier on
+// the shape's LPs and keep the faster; the default (first tier used) stays unless skipping
+// it is more than 3% faster.  Workloads whose final bases never fit the first tier pay a
+// pass that only hands every LP on.
+static int tune_check(blpg_ctx* ctx, std::vector<blpg_lp>& lps, int prec, int m, int n,
+                      int reps) {
+  const int tsz = prec == BLPG_FP32 ? 4 : 8;
+  ctx->tuner.erase_check(prec, m, n);
+  if (!verify_tier_used(0, m, n, tsz) || verify_tier_covers(0, m, n, tsz)) return BLPG_OK;
+  double t[2] = {1e30, 1e30};
+  for (int skip = 0; skip < 2; ++skip) {
+    ctx->tuner.set_check(prec, m, n, skip);
+    blpg_batch* B = nullptr;
+    int rc = blpg_batch_upload(ctx, lps.data(), (int64_t)lps.size(), &B);
+    for (int r = 0; r <= reps && !rc; ++r) {
+      rc = blpg_batch_solve(ctx, B);
+      if (!rc) rc = blpg_batch_sync(ctx);
+      blpg_batch_info info;
+      if (!rc) rc = blpg_batch_info_get(ctx, B, &info);
+      if (!rc && r > 0 && info.verify_ms > 0) t[skip] = std::min(t[skip], info.verify_ms);
+    }
+    if (B) blpg_batch_free(ctx, B);
+    if (rc) {
+      ctx->tuner.erase_check(prec, m, n);
+      return rc;
+    }
+  }
+  const int skip = t[1] < 0.97 * t[0] ? 1 : 0;
+  if (t[skip] < 1e30) ctx->tuner.set_check(prec, m, n, skip, t[skip]);
+  else ctx->tuner.erase_check(prec, m, n);
+  return BLPG_OK;
+}
+
 int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps) {
   if (!ctx || (count > 0 && !lps) || count < 0)
     return fail(ctx, BLPG_E_INVALID, "invalid arguments to blpg_autotune");
@@ -1301,6 +1336,10 @@ int blpg_autotune(blpg_ctx* ctx, const blpg_lp* lps, int64_t count, int32_t reps
       break;
     }
     if (best.first >= 0) ctx->tuner.set_preference(prec, m, n, best.first, best.second, best_ms);
+    if (ctx->opt.verify || prec == BLPG_FP32) {
+      rc = tune_check(ctx, g.second, prec, m, n, reps);
+      if (rc) break;
+    }
   }
   ctx->opt = saved;
   return rc;

@@ -50,7 +50,17 @@ class Tuner {
     pref_[key(precision, m, n)] = {kernel, param};
     ms_[key(precision, m, n)] = ms;
   }
-  void clear_preferences() { pref_.clear(); ms_.clear(); }
+  void clear_preferences() { pref_.clear(); ms_.clear(); chk_.clear(); }
+  // check plan of a shape (measured by the autotuner): bit 0 skips the first check tier, so
+  // the shape's LPs go straight to the next (for workloads whose final bases never fit it)
+  void set_check(int precision, int m, int n, int flags, double ms = -1.0) {
+    chk_[key(precision, m, n)] = {flags, ms};
+  }
+  int check_flags(int precision, int m, int n) const {
+    const auto it = chk_.find(key(precision, m, n));
+    return it == chk_.end() ? 0 : it->second.first;
+  }
+  void erase_check(int precision, int m, int n) { chk_.erase(key(precision, m, n)); }
   // the stored preference of a shape (false: none)
   bool get_preference(int precision, int m, int n, int& kernel, int& param, double& ms) const {
     const auto it = pref_.find(key(precision, m, n));
@@ -65,7 +75,8 @@ class Tuner {
     pref_.erase(key(precision, m, n));
     ms_.erase(key(precision, m, n));
   }
-  // "precision m n kernel param ms" lines
+  // "precision m n kernel param ms" lines; kernel 0: the shape's check plan (param: flags,
+  // ms: the check's measured time)
   std::string export_text() const {
     std::string out;
     char line[128];
@@ -75,6 +86,13 @@ class Tuner {
       const auto it = ms_.find(k);
       snprintf(line, sizeof line, "%d %d %d %d %d %.6f\n", p, m, n, kv.second.first,
                kv.second.second, it == ms_.end() ? -1.0 : it->second);
+      out += line;
+    }
+    for (const auto& kv : chk_) {
+      const long k = kv.first;
+      const int p = (int)(k >> 40), m = (int)((k >> 20) & 0xFFFFF), n = (int)(k & 0xFFFFF);
+      snprintf(line, sizeof line, "%d %d %d 0 %d %.6f\n", p, m, n, kv.second.first,
+               kv.second.second);
       out += line;
     }
     return out;
@@ -89,9 +107,14 @@ class Tuner {
         if (!*p) break;
         return false;
       }
-      if (pr < 0 || pr > 1 || m < 1 || n < 1 || m > 4096 || n > 4096 || kern < 1 || kern > 4)
+      if (pr < 0 || pr > 1 || m < 1 || n < 1 || m > 4096 || n > 4096 || kern < 0 || kern > 4)
         return false;
-      set_preference(pr, m, n, kern, param, ms);
+      if (kern == 0) {
+        if (param < 0 || param > 1) return false;
+        set_check(pr, m, n, param, ms);
+      } else {
+        set_preference(pr, m, n, kern, param, ms);
+      }
       p += used;
     }
     return true;
@@ -190,6 +213,7 @@ class Tuner {
   size_t lds_ = 65536;
   std::map<long, std::pair<int, int>> pref_;
   std::map<long, double> ms_;
+  std::map<long, std::pair<int, double>> chk_;  // check plan flags, measured check time
 };
 
 }  // namespace blpg
