@@ -62,3 +62,25 @@ def test_autotune_table_and_results():
         S2.tune_set(200, 200, "tile", 0)  # variant 0 holds 30x38 at most
     S.close()
     S2.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_check_plan_skip_first_tier_same_decisions(precision):
+    # A shape whose check plan skips the first LDS tier (as the autotuner chooses for final
+    # bases that never fit it) hands its LPs straight to the next tier: every decision,
+    # objective and KKT measure is bit-identical to the default plan (both tiers stage the
+    # same blocks and run the same sweeps), and the table row round-trips.
+    lps = sa.gen_config(2, 2, 105, 103, 512, shape_idx=3, batch_count=1024)
+    S = sa.Solver(precision=precision, verify=True)
+    base = S.solve(lps)
+    p = 1 if precision == "fp32" else 0
+    S.tune_import(f"{p} 105 103 0 1 0.25\n")
+    assert (p, 105, 103, 0, 1, 0.25) in S.tune_table()
+    r = S.solve(lps)
+    for f in ("status", "fallback", "pivots"):
+        assert np.array_equal(getattr(base, f), getattr(r, f)), f
+    for f in ("objective", "kkt_primal", "kkt_dual"):
+        assert np.array_equal(getattr(base, f), getattr(r, f), equal_nan=True), f
+    with pytest.raises(sa.BlpgError):
+        S.tune_import(f"{p} 105 103 0 7 0.25\n")  # unknown check-plan flags
+    S.close()
