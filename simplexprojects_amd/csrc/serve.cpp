@@ -323,12 +323,15 @@ void worker_loop(blpg_server* S) {
         Shape& sh = *kv.second;
         bool any_busy = false;
         for (auto& a : sh.arenas) any_busy = any_busy || a->busy;
-        // the filling arena (its requests are older than any in the overflow queue)
-        if (sh.fill >= 0) {
-          Arena* fa = sh.arenas[sh.fill].get();
+        // arenas holding requests (older than any in the overflow queue): the filling one,
+        // and any the filling moved on from when it became full before this loop launched
+        // it (a submit picks a new empty arena then), which is launched at once
+        for (size_t ai = 0; ai < sh.arenas.size(); ++ai) {
+          Arena* fa = sh.arenas[ai].get();
+          if (fa->busy || fa->filled.empty()) continue;
+          const bool filling = (int)ai == sh.fill;
           const bool full = (int)fa->filled.size() >= S->cfg.max_batch;
-          if (!fa->busy && !fa->filled.empty() &&
-              (full || !sh.queue.empty() || !any_busy ||
+          if ((full || !filling || !sh.queue.empty() || !any_busy ||
                now - fa->filled.front().t0 >= max_wait) &&
               fa->copied.load(std::memory_order_acquire) == (int)fa->filled.size()) {
             fa->busy = true;
@@ -336,7 +339,7 @@ void worker_loop(blpg_server* S) {
             fa->inflight = std::move(fa->filled);
             fa->filled.clear();
             fa->copied.store(0, std::memory_order_relaxed);
-            sh.fill = -1;
+            if (filling) sh.fill = -1;
             ready_fill.push_back(fa);
           }
         }

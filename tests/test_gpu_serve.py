@@ -76,6 +76,25 @@ def test_server_poisson_replay():
     S.close()
 
 
+def test_server_burst_launches_every_full_arena():
+    # Back-to-back submissions into arenas of 8 slots: an arena fills up and the next request
+    # opens another arena of the shape before the worker has launched the full one.  The
+    # worker launches every arena that holds requests, so all of them complete and match
+    # the batch solver (regression: a full arena the filling had moved on from was never
+    # launched, and its requests waited until the replay's stall report).
+    lps = mixed([320, 320, 320, 320])
+    ref = sa.Solver(precision="fp64").solve(lps)
+    S = sa.Solver(precision="fp64")
+    srv = sa.Server(S, max_batch=8, max_wait_us=1000, precision="fp64")
+    for rnd in range(3):
+        res = srv.replay(lps, np.zeros(len(lps)))
+        assert np.array_equal(res["status"], ref.status), rnd
+        assert np.array_equal(res["pivots"], ref.pivots), rnd
+    assert srv.stats()["completed"] == 3 * len(lps)
+    srv.close()
+    S.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_server_tagged_warmstart_50k(precision):
     """The serving workload at its offered rate (Poisson 50k LPs/s, the five shapes, 64 GB
