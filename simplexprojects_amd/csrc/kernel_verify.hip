@@ -58,11 +58,16 @@ constexpr int kRB = 8;  // rows per wavefront pass of the global-memory row swee
 
 __host__ __device__ inline int part_cap(int mcap, int nt) { return mcap + 1 > nt ? mcap + 1 : nt; }
 
-// per-LP vectors (LDS when they fit, else a slab per workgroup in HBM) for NT threads
+// per-LP vectors (LDS when they fit, else a slab per workgroup in HBM) for NT threads.
+// The double arrays are 16-byte aligned, the int arrays (read one element at a time) only
+// 4-byte aligned, and the total is rounded to 16 for the staged blocks that follow: at
+// 105x103 the 16-byte padding of the int arrays alone kept LPs with 16 or 17 basic
+// structurals out of the 31 KB tier.
 struct VLayout {
   size_t x, r, y, rho, d, w, bp, ae, cS, part, bs, rowof, colp, S, Srow, Pn, Pcol, total;
   __host__ __device__ VLayout(int mcap, int ncap, int nt) {
     auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
+    auto al4 = [](size_t v) { return (v + 3) & ~(size_t)3; };
     size_t o = 0;
     x = o; o += al(8 * (size_t)mcap);      // basic values by row
     r = o; o += al(8 * (size_t)mcap);      // residual by constraint
@@ -75,14 +80,14 @@ struct VLayout {
     cS = o; o += al(8 * (size_t)mcap);     // c of the basic structurals (S order)
     part = o;                              // column-sum partials (part_cap doubles)
     o += al(8 * (size_t)part_cap(mcap, nt));
-    bs = o; o += al(4 * (size_t)mcap);     // basic variable of each row
-    rowof = o; o += al(4 * (size_t)(mcap + ncap + 1));  // row of each basic variable, -1
-    colp = o; o += al(4 * (size_t)mcap);   // tableau column of each nonbasic slack, -1
-    S = o; o += al(4 * (size_t)mcap);      // basic structurals (ascending)
-    Srow = o; o += al(4 * (size_t)mcap);   // their rows
-    Pn = o; o += al(4 * (size_t)mcap);     // nonbasic slacks (ascending constraint index)
-    Pcol = o; o += al(4 * (size_t)mcap);   // their tableau columns
-    total = o;
+    bs = o; o += al4(4 * (size_t)mcap);    // basic variable of each row
+    rowof = o; o += al4(4 * (size_t)(mcap + ncap + 1));  // row of each basic variable, -1
+    colp = o; o += al4(4 * (size_t)mcap);  // tableau column of each nonbasic slack, -1
+    S = o; o += al4(4 * (size_t)mcap);     // basic structurals (ascending)
+    Srow = o; o += al4(4 * (size_t)mcap);  // their rows
+    Pn = o; o += al4(4 * (size_t)mcap);    // nonbasic slacks (ascending constraint index)
+    Pcol = o; o += al4(4 * (size_t)mcap);  // their tableau columns
+    total = al(o);
   }
 };
 
