@@ -64,15 +64,29 @@ def test_autotune_table_and_results():
     S2.close()
 
 
+@pytest.mark.parametrize("workload", ["mixed", "families"])
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_check_plan_skip_first_tier_same_decisions(precision):
+def test_check_plan_skip_first_tier_same_decisions(precision, workload):
     # A shape whose check plan skips the first LDS tier (as the autotuner chooses for final
     # bases that never fit it) hands its LPs straight to the next tier: every decision,
     # objective and KKT measure is bit-identical to the default plan (both tiers stage the
-    # same blocks and run the same sweeps), and the table row round-trips.
-    lps = sa.gen_config(2, 2, 105, 103, 512, shape_idx=3, batch_count=1024)
+    # same blocks and run the same sweeps), and the table row round-trips.  "mixed": the
+    # benchmark's 105x103 LPs (33-71 basic structurals: none fit the first tier, which only
+    # defers them); "families": parametric-family LPs (8-18 basic structurals at the end;
+    # the first tier holds up to 17, including the 16-17 that only fit since its int
+    # vectors are 4-byte aligned), checked by the first tier under the default plan and by
+    # the second under the other.
+    if workload == "mixed":
+        lps = sa.gen_config(2, 2, 105, 103, 512, shape_idx=3, batch_count=1024)
+    else:
+        lps = []
+        for f in range(4):
+            lps += sa.gen_config(4, 4, 105, 103, 128, shape_idx=f, first=512, batch_count=1024)
     S = sa.Solver(precision=precision, verify=True)
     base = S.solve(lps)
+    if workload == "families":
+        ks = np.array([int((np.asarray(bb) < 103).sum()) for bb in base.basis])
+        assert ((ks >= 16) & (ks <= 17)).any(), np.bincount(ks)
     p = 1 if precision == "fp32" else 0
     S.tune_import(f"{p} 105 103 0 1 0.25\n")
     assert (p, 105, 103, 0, 1, 0.25) in S.tune_table()
